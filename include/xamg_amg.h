@@ -1,0 +1,61 @@
+/*
+ * libxamg.so -- host-side setup: the multigrid hierarchy the reference gets from hypre
+ * BoomerAMG (src/hypre/detail/hypre_hierarchy.inl:31-52, hypre_base.inl:337-403), the
+ * reference's test-matrix generator (examples/common/generator/cube.inl:28-174) and the
+ * right-hand-side generator of the benchmarks.  Setup runs once, on the host.
+ * All functions return 0 on success, < 0 on failure.
+ */
+#ifndef XAMG_AMG_H_
+#define XAMG_AMG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "xamg_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xamg_amg_s *xamg_amg;
+
+/* the MultiGrid keys of src/param/xamg_params_details.h:179-209 that the classical
+ * setup honours, with the reference defaults in comments */
+typedef struct {
+    float strength;     /* mg_strength_threshold 0.5 */
+    float max_row_sum;  /* mg_max_row_sum 0.9 */
+    int32_t coarse_size;/* mg_coarse_matrix_size 100 */
+    int32_t max_levels; /* mg_max_levels 25 */
+    float trunc_factor; /* mg_trunc_factor 0.25 */
+    int32_t pmax;       /* mg_Pmax_elements 4 */
+    int32_t second_pass;/* Ruge-Stueben second pass (0) */
+    int32_t eig_iters;  /* power iterations for Chebyshev bounds (10) */
+    uint64_t eig_seed;
+} xamg_amg_params;
+
+void xamg_amg_default_params(xamg_amg_params *p);
+int xamg_amg_setup(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                   const xamg_amg_params *p, xamg_amg *out);
+int xamg_amg_nlevels(xamg_amg H);
+/* which: 0 A_l, 1 P_l, 2 R_l.  Pointers stay valid until xamg_amg_free. */
+int xamg_amg_level(xamg_amg H, int level, int which, int64_t *nrows, int64_t *ncols,
+                   int64_t *nnz, const uint32_t **row, const uint32_t **col,
+                   const double **val);
+int xamg_amg_eigs(xamg_amg H, int level, double *min_eig, double *max_eig);
+/* re-estimate lambda_max of D^-1/2 A D^-1/2 on every level by `iters` power iterations */
+int xamg_amg_estimate_eigs(xamg_amg H, int iters, uint64_t seed);
+/* copy into a device-side hierarchy object of libxamg_hip */
+int xamg_amg_to_device(xamg_amg H, xh_hier *out);
+int xamg_amg_free(xamg_amg H);
+
+/* 7-point cube Laplacian (cube.inl:84-146), one block; nnz of the result first */
+int64_t xamg_cube_nnz(int64_t nx, int64_t ny, int64_t nz);
+int xamg_cube(int64_t nx, int64_t ny, int64_t nz, uint32_t *row, uint32_t *col, double *val);
+/* RHS entries uniform in [-1,1]: b[i] = 2*u(splitmix64(seed, i)) - 1, i = row*nv + k */
+int xamg_rhs_uniform(uint64_t seed, int64_t count, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,158 @@
+/*
+ * libxamg_hip.so -- thin C ABI over the MI355X (gfx950) solve phase of XAMG.
+ *
+ * The reference XAMG has no GPU path; its solve phase is the C++ template stack
+ *   XAMG_solver_solve2 (api/c/xamg_c_api.cpp:522-531)
+ *     -> base_solver<F,NV>::solve (src/solvers/solver.h:289-311)
+ *       -> MergedPBiCGStab / PBiCGStab / BiCGStab / MergedBiCGStab::solve
+ *          with the MultiGrid preconditioner (src/solvers/detail/mg.inl:166-334)
+ *         -> blas2::Ax_y / Axpy / SGS (src/blas2/blas2.h:185-326) and the merged BLAS-1
+ *            groups (src/blas/detail/specific/blas_merged_bicgstab.inl:33-257).
+ * Each entry point below names the reference interface it replaces.  All functions
+ * return 0 on success and a negative code on failure (xh_last_error() gives the
+ * message); nothing aborts.  Plain pointers and sizes only.  Vectors are the
+ * reference's NV-interleaved layout x[i*nv + k] (src/primitives/vector/vector.h:509-526).
+ * Supported nv: 1, 2, 4, 8, 16, 32, 64.
+ *
+ * Threading: one host thread per solver object; objects are not thread-safe.
+ */
+#ifndef XAMG_HIP_H_
+#define XAMG_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xh_matrix_s *xh_matrix;
+typedef struct xh_hier_s *xh_hier;
+typedef struct xh_solver_s *xh_solver;
+
+/* error codes */
+#define XH_OK 0
+#define XH_EINVAL -1
+#define XH_EHIP -2
+#define XH_ENOTSUP -3
+#define XH_ENOMEM -4
+
+/* method ids (the reference's registry strings, src/solvers/solver.h:408-443) */
+#define XH_IDENTITY 0
+#define XH_JACOBI 1
+#define XH_HSGS 2
+#define XH_CHEBYSHEV 3
+#define XH_DIRECT 4
+#define XH_MULTIGRID 5
+#define XH_BICGSTAB 6
+#define XH_PBICGSTAB 7
+#define XH_MERGED_BICGSTAB 8
+#define XH_MERGED_PBICGSTAB 9
+#define XH_NONE -1
+
+/* Runtime init: replaces XAMG::init's device part (src/init.h:54-76; the reference's
+ * sys/gpu_wrapper.cu only enumerates devices).  Selects the HIP device. */
+int xh_init(int device);
+int xh_finalize(void);
+const char *xh_last_error(void);
+/* device name, compute units, HBM bytes */
+int xh_device_info(char *name, int name_len, int *compute_units, double *hbm_bytes);
+
+/* Device memory helpers (the reference copies user buffers on upload/download,
+ * api/c/xamg_c_api.cpp:350-373). */
+int xh_malloc(void **dptr, size_t bytes);
+int xh_free(void *dptr);
+int xh_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int xh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int xh_synchronize(void);
+
+/* CSR upload: replaces XAMG_csr_matrix_alloc/fill + XAMG_matrix_construct
+ * (api/c/xamg_c_api.cpp:404-480).  Row/col are u32 (XAMG_LIMIT_TYPES_USAGE),
+ * values f64; prec 0 keeps f64, 1 stores f32 (matrix_block::assemble with reduced_prec,
+ * src/primitives/matrix/matrix.h:79-93).  index_mode: 0 auto (u16 panel-relative
+ * columns when every panel of >= 64 rows spans <= 65536 columns), 1 u32, 2 u16 (fails
+ * if impossible).  Rows are re-ordered diagonal first, then ascending column. */
+int xh_matrix_create(int64_t nrows, int64_t ncols, int64_t nnz, const uint32_t *row,
+                     const uint32_t *col, const double *val, int prec, int index_mode,
+                     xh_matrix *out);
+/* index_bits: 32 or 16; panel_rows: rows per u16 panel (0 for u32) */
+int xh_matrix_info(xh_matrix A, int64_t *nrows, int64_t *ncols, int64_t *nnz, int *prec,
+                   int *index_bits, int64_t *panel_rows, size_t *device_bytes);
+int xh_matrix_destroy(xh_matrix A);
+
+/* SpMV on device vectors: y = A x (accumulate 0, backend::csr_Ax_y,
+ * src/blas2/detail/csr_matvec.inl:149-260) or y += A x (accumulate 1, backend::csr_Axpy,
+ * :33-144).  x, y have the matrix's precision. */
+int xh_spmv(xh_matrix A, int nv, const void *dx, void *dy, int accumulate);
+/* Same launch repeated `reps` times on the library stream, timed with HIP events on
+ * that stream: average milliseconds per launch. */
+int xh_spmv_timed(xh_matrix A, int nv, const void *dx, void *dy, int accumulate, int reps,
+                  double *ms_per_launch);
+/* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
+int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);
+
+/* One parameter role of the reference param dictionary
+ * (src/param/xamg_params_details.h:79-236). */
+typedef struct {
+    int32_t method;
+    int32_t max_iters;
+    float abs_tol, rel_tol;
+    int32_t conv_check;
+    float relax_factor;
+    int32_t poly_order;
+    float spectrum_fraction;
+} xh_role;
+
+typedef struct {
+    xh_role solver, precond, pre, post, coarse;
+    int32_t mg_cycle;        /* 0 V, 1 W, 2 F */
+    int32_t mg_prec;         /* 0 fp64; 1 mg_reduced_precision (levels >= 1 fp32);
+                                2 fp32 preconditioner including level 0 */
+    int32_t hsgs_ncores;     /* HSGS block count when hsgs_block_rows == 0 */
+    int32_t hsgs_block_rows; /* > 0: per-level blocks ceil(n/hsgs_block_rows) */
+    int32_t coarse_fp64;     /* mg_prec 2: coarsest direct solve in fp64 */
+    int32_t index_mode;      /* column index width of the finest operator, as above */
+    int32_t graph;           /* 1: replay one captured iteration as a hipGraph */
+    int32_t reserved[4];
+} xh_params;
+
+/* Hierarchy (the output of hypre::get_full_hierarchy, hypre_hierarchy.inl:31-52):
+ * level l holds A_l and, for l < nlevels-1, P_l (n_l x n_{l+1}) and R_l = P_l^T.
+ * Arrays are copied. */
+int xh_hier_create(int nlevels, xh_hier *out);
+int xh_hier_set_level(xh_hier H, int level, int64_t n, int64_t nnzA, const uint32_t *rowA,
+                      const uint32_t *colA, const double *valA, int64_t nc, int64_t nnzP,
+                      const uint32_t *rowP, const uint32_t *colP, const double *valP,
+                      int64_t nnzR, const uint32_t *rowR, const uint32_t *colR,
+                      const double *valR, double min_eig, double max_eig);
+int xh_hier_destroy(xh_hier H);
+
+/* Solver construction: replaces construct_solver_hierarchy<F,NV>
+ * (src/solvers/solver.h:481-500).  Uploads the hierarchy once, in the precision the
+ * parameters ask for, and allocates every device buffer for `nv` right-hand sides. */
+int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out);
+/* host vectors b, x (x holds x0 on entry, the solution on return):
+ * replaces XAMG_solver_solve2 (api/c/xamg_c_api.cpp:522-531) */
+int xh_solver_solve(xh_solver S, const double *b, double *x);
+/* device vectors (inputs resident in HBM): the timed path */
+int xh_solver_solve_device(xh_solver S, const double *db, double *dx);
+/* solver_stats (src/solvers/solver.h:48-66) */
+int xh_solver_stats(xh_solver S, int32_t *iters, int32_t *converged, double *abs_res,
+                    double *rel_res);
+/* sqrt of the tested residual per convergence check: hist[check*nv + k] */
+int xh_solver_history(xh_solver S, double *hist, int32_t max_checks, int32_t *nchecks);
+/* one preconditioner application x = M^{-1} b on device vectors (x starts at zero) */
+int xh_solver_precond_apply(xh_solver S, const double *db, double *dx);
+/* renew the solver-role parameters (tolerances, max_iters, convergence_check) of an
+ * existing solver: base_solver::renew_params (src/solvers/solver.h:277-288); the method
+ * itself cannot change */
+int xh_solver_renew(xh_solver S, const xh_role *solver);
+/* device memory held by the solver (matrices + vectors) */
+int xh_solver_info(xh_solver S, int *nlevels, size_t *device_bytes);
+int xh_solver_destroy(xh_solver S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

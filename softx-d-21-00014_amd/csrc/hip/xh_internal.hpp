@@ -1,0 +1,139 @@
+// Internal interface between the HIP kernel files and the runtime / C-ABI layer.
+// Everything here is gfx950 (CDNA4) code: wave64, 256-thread workgroups, workgroups
+// mapped XCD-contiguously (8 XCDs, blockIdx dealt round-robin across them).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace xh {
+
+constexpr int kBlock = 256;
+constexpr int kMaxNV = 64;
+constexpr int kMaxPartials = 1024; // workgroups that write dot partials
+
+enum Prec : int { F64 = 0, F32 = 1 };
+enum Idx : int { IDX_U32 = 0, IDX_U16 = 1 }; // u16: panel-relative column (col = base + u16)
+
+// A device CSR matrix (rows stored diagonal first, ascending columns after it).
+struct Mat {
+    int64_t nrows = 0, ncols = 0, nnz = 0;
+    int prec = F64;
+    int idx = IDX_U32;
+    int panel_shift = 0;       // rows per column panel = 1 << panel_shift (IDX_U16)
+    uint32_t *row = nullptr;   // nrows+1
+    void *col = nullptr;       // nnz of uint32_t or uint16_t
+    uint32_t *panel_base = nullptr; // one base column per panel (IDX_U16)
+    void *val = nullptr;       // nnz of double or float
+    void *inv_diag = nullptr;      // nrows, prec of the matrix
+    void *inv_sqrt_diag = nullptr; // nrows, prec of the matrix
+    size_t bytes = 0;          // device bytes owned
+};
+
+// Scalar state of a Krylov solve, resident in device memory.  One slot per RHS.
+struct SolverState {
+    double rho[kMaxNV], rho0[kMaxNV], alpha[kMaxNV], nalpha[kMaxNV], omega[kMaxNV];
+    double nomega[kMaxNV], beta[kMaxNV], ac[kMaxNV], oc[kMaxNV], pcoef[kMaxNV];
+    double phi[kMaxNV], psi[kMaxNV], theta[kMaxNV], res[kMaxNV], scal1[kMaxNV];
+    double iconv[kMaxNV], abs_res[kMaxNV], rel_res[kMaxNV];
+    int conv[kMaxNV];
+    int iters;    // reference stats.iters
+    int done;     // solve finished (converged or max_iters): every later kernel is a no-op
+    int conv_all; // the last convergence check found every RHS converged
+    int err;      // 1: division by a zero-flagged value (reference assert)
+    // parameters (copied once per solve)
+    int nv, max_iters, conv_check, hist_len;
+    float abs_tol, rel_tol;
+    double *hist; // [hist_len][nv] sqrt(res) per convergence check, or nullptr
+};
+
+// ---- SpMV family (xh_spmv.hip) ----
+enum SpmvOp : int {
+    OP_SET = 0,      // y = A x                                   (csr_Ax_y)
+    OP_ADD = 1,      // y += A x, accumulated into y              (csr_Axpy)
+    OP_RESID = 2,    // y = b - A x  (optionally stored as fp32)  (Ax_y + axpby(1,b,-1,y))
+    OP_DOT1 = 3,     // y = A x and partial (y, o1)
+    OP_DOT3 = 4,     // y = A x and partials (y,o1), (y,y), (o1,o1)
+    OP_CHEB0 = 5,    // r = isd*(b - A x); u = (c*r)*isd  [or x += (c*r)*isd when last]
+    OP_CHEB = 6,     // t1 = c*r + (A u)*isd; u' = t1*isd [or x (+)= t1*isd when last]
+    OP_PROLONG32 = 7,// x64 = (double)(P32 xH32) + x64
+    OP_OFFBLOCK = 8, // y = b - (off-block part of A) x, HSGS block partition
+    OP_RESID0 = 9,   // y = b - A x, b and y f64, written both f64 and group1-fused
+};
+
+struct SpmvArgs {
+    const void *b = nullptr;    // RESID/CHEB0/OFFBLOCK right-hand side (matrix prec)
+    const void *o1 = nullptr;   // DOT1/DOT3 second vector
+    const void *isd = nullptr;  // CHEB*: inverse sqrt diagonal
+    const void *r = nullptr;    // CHEB: scaled residual r
+    void *out2 = nullptr;       // CHEB0: r output; RESID0: p;
+    void *out3 = nullptr;       // RESID0: r
+    void *xupd = nullptr;       // CHEB*: x to update when `last`
+    double c = 0.0;             // CHEB*: polynomial coefficient (cast to the matrix prec)
+    int last = 0;               // CHEB*: this step ends the polynomial (updates x)
+    int x_zero = 0;             // CHEB*: x is zero-flagged (x = t1 instead of x + t1)
+    int out_f32 = 0;            // RESID: store y as float (mixed-precision restriction)
+    double *partials = nullptr; // DOT*: [grid][ndot*nv]
+    const int *done = nullptr;  // skip the launch when *done
+    int64_t blk_rows = 0;       // OFFBLOCK: HSGS core block size (rows)
+    int grid_cap = 0;           // 0: default
+};
+
+// returns the number of partial blocks written (DOT ops) or 0
+int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
+         hipStream_t s);
+
+// ---- vector kernels, finalize, smoothers (xh_vec.hip) ----
+enum VecOp : int {
+    V_S_UPDATE = 0,   // s = r + nalpha*v
+    V_XR_UPDATE = 1,  // x = ac*p_ + oc*s_ + x ; r = s - omega*t ; partial (r, r0)
+    V_P_UPDATE = 2,   // p = r + pcoef*v + beta*p
+    V_GROUP1 = 3,     // r0 = b - r0 ; p = r = r0 ; partial (r0, r0)
+    V_X_UPDATE = 4,   // x = ac*p_ + oc*s_ + x           (BiCGStab x before check)
+    V_R_UPDATE = 5,   // r = s - omega*t ; partial (r, r0)
+    V_GROUP5 = 6,     // x += ac*p + oc*s ; p = r + beta*p - (beta*omega)*v   (merged)
+    V_DOT3 = 7,       // partials (t,s), (t,t), (s,s) of existing vectors
+    V_DOT1 = 8,       // partial (a, b)
+};
+
+struct VecArgs {
+    void *x = nullptr, *p = nullptr, *s = nullptr, *r = nullptr, *r0 = nullptr;
+    const void *v = nullptr, *t = nullptr, *b = nullptr, *p_ = nullptr, *s_ = nullptr;
+    SolverState *st = nullptr;
+    double *partials = nullptr;
+    int gate = 0; // 0: run unless done; 1: run only if the last check converged every RHS
+};
+
+int vec_op(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s);
+
+enum FinOp : int {
+    FIN_INIT = 0,   // rho0 = sum ; convergence check on (rho0, rho0) ; rho = rho0
+    FIN_ALPHA = 1,  // ++iters ; scal1 = sum ; alpha = rho / scal1 ; nalpha = -alpha
+    FIN_OMEGA = 2,  // phi, psi, theta ; omega ; res ; ac, oc ; convergence check
+    FIN_BETA = 3,   // rho_new ; beta ; pcoef ; rho = rho_new ; max_iters check
+    FIN_OMEGA_MERGED = 4, // as FIN_OMEGA, but ac/oc recomputed with the updated iconv
+};
+void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int nv,
+              hipStream_t s);
+
+// smoother / transfer helpers
+// `done` (nullable): device flag; when set the launch is a no-op
+void cheb_init_zero(int64_t n, int nv, int prec, const void *b, const void *isd, double c,
+                    void *r, void *u_or_x, int last, const int *done, hipStream_t s);
+void jacobi_update(int64_t n, int nv, int prec, void *x, const void *b, const void *r,
+                   const void *inv_diag, double relax, int x_zero, const int *done,
+                   hipStream_t s);
+void hsgs_sweeps(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax,
+                 int64_t blk_rows, const int *done, hipStream_t s);
+int64_t hsgs_block_rows(int64_t n, int nblocks);
+void dense_matvec(int64_t n, int nv, int prec, const void *D, const void *x, void *y,
+                  const int *done, hipStream_t s);
+void convert(int64_t count, int from_prec, const void *src, void *dst, const int *done,
+             hipStream_t s);
+void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *done,
+              hipStream_t s);
+void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s);
+
+} // namespace xh

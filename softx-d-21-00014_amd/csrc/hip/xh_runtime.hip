@@ -1,0 +1,1239 @@
+// C ABI of libxamg_hip.so: device objects, hierarchy upload and the GPU solve driver.
+//
+// The solve driver restates the reference solve loops
+//   PBiCGStab::solve        src/solvers/detail/pbicgstab.inl:44-237
+//   MergedPBiCGStab::solve  src/solvers/detail/merged/merged_pbicgstab.inl:44-225
+//   BiCGStab::solve         src/solvers/detail/bicgstab.inl:44-221
+//   MergedBiCGStab::solve   src/solvers/detail/merged/merged_bicgstab.inl:44-208
+//   MultiGrid::MG_cycle     src/solvers/detail/mg.inl:166-260
+//   Chebyshev / Jacobi / HSGS / Direct smoothers (chebyshev.inl, jacobi.inl, hsgs.inl,
+//   direct.inl)
+// as a stream of fused kernels.  All Krylov scalars live in device memory
+// (SolverState) and the convergence test runs in a one-workgroup kernel, so the host
+// never waits inside an iteration: it enqueues whole iterations (or replays one
+// captured hipGraph per iteration) and polls the device `done` flag one iteration
+// behind.  Kernels launched after `done` is set return immediately, so x is frozen at
+// exactly the iteration where the reference returns.
+#include "../../../include/xamg_hip.h"
+#include "xh_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace xh {
+namespace {
+
+thread_local std::string g_err;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define XH_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(XH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+bool nv_ok(int nv) {
+    return nv == 1 || nv == 2 || nv == 4 || nv == 8 || nv == 16 || nv == 32 || nv == 64;
+}
+
+int ensure_init() {
+    if (g_device >= 0)
+        return 0;
+    return xh_init(0);
+}
+
+// ---- host CSR ----
+struct HostCsr {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<uint32_t> row, col;
+    std::vector<double> val;
+};
+
+void sort_diag_first(HostCsr &m) {
+    for (int64_t i = 0; i < m.nrows; ++i) {
+        uint32_t b = m.row[i], e = m.row[i + 1];
+        std::vector<std::pair<uint32_t, double>> r;
+        r.reserve(e - b);
+        for (uint32_t j = b; j < e; ++j)
+            r.emplace_back(m.col[j], m.val[j]);
+        bool sorted = true;
+        for (size_t q = 0; q < r.size() && sorted; ++q) {
+            if (q == 0)
+                continue;
+            bool d0 = r[q - 1].first == (uint32_t)i, d1 = r[q].first == (uint32_t)i;
+            if (d1 || (!d0 && r[q - 1].first > r[q].first))
+                sorted = false;
+        }
+        if (sorted)
+            continue;
+        std::sort(r.begin(), r.end(), [i](const std::pair<uint32_t, double> &x,
+                                          const std::pair<uint32_t, double> &y) {
+            bool dx = x.first == (uint32_t)i, dy = y.first == (uint32_t)i;
+            if (dx != dy)
+                return dx;
+            return x.first < y.first;
+        });
+        for (uint32_t j = b; j < e; ++j) {
+            m.col[j] = r[j - b].first;
+            m.val[j] = r[j - b].second;
+        }
+    }
+}
+
+// largest panel shift (rows per panel = 2^s, s >= 6) whose every panel spans at most
+// 65536 columns; -1 when none does
+int find_panel_shift(const HostCsr &m) {
+    for (int s = 20; s >= 6; --s) {
+        const int64_t pr = (int64_t)1 << s;
+        bool ok = true;
+        for (int64_t p0 = 0; p0 < m.nrows && ok; p0 += pr) {
+            const int64_t p1 = std::min(m.nrows, p0 + pr);
+            uint32_t lo = UINT32_MAX, hi = 0;
+            for (uint32_t j = m.row[p0]; j < m.row[p1]; ++j) {
+                lo = std::min(lo, m.col[j]);
+                hi = std::max(hi, m.col[j]);
+            }
+            if (m.row[p1] > m.row[p0] && (uint64_t)hi - lo > 65535u)
+                ok = false;
+        }
+        if (ok)
+            return s;
+    }
+    return -1;
+}
+
+template <typename T>
+int dmalloc(T **p, size_t count, size_t &acc) {
+    if (count == 0) {
+        *p = nullptr;
+        return 0;
+    }
+    XH_HIP(hipMalloc((void **)p, count * sizeof(T)));
+    acc += count * sizeof(T);
+    return 0;
+}
+
+int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
+    M = Mat();
+    M.nrows = h.nrows;
+    M.ncols = h.ncols;
+    M.nnz = (int64_t)h.col.size();
+    M.prec = prec;
+    const int64_t n = h.nrows, nnz = M.nnz;
+    int shift = -1;
+    if (index_mode != 1)
+        shift = find_panel_shift(h);
+    if (index_mode == 2 && shift < 0)
+        return fail(XH_EINVAL, "u16 column panels impossible for this matrix");
+    M.idx = shift >= 0 ? IDX_U16 : IDX_U32;
+    M.panel_shift = shift >= 0 ? shift : 0;
+    size_t acc = 0;
+    int rc;
+    if ((rc = dmalloc(&M.row, n + 1, acc)))
+        return rc;
+    XH_HIP(hipMemcpy(M.row, h.row.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+    if (M.idx == IDX_U16) {
+        const int64_t pr = (int64_t)1 << shift;
+        const int64_t np = (n + pr - 1) / pr;
+        std::vector<uint32_t> base(np, 0);
+        std::vector<uint16_t> c16(nnz);
+        for (int64_t p = 0; p < np; ++p) {
+            const int64_t p0 = p * pr, p1 = std::min(n, p0 + pr);
+            uint32_t lo = UINT32_MAX;
+            for (uint32_t j = h.row[p0]; j < h.row[p1]; ++j)
+                lo = std::min(lo, h.col[j]);
+            base[p] = lo == UINT32_MAX ? 0 : lo;
+            for (uint32_t j = h.row[p0]; j < h.row[p1]; ++j)
+                c16[j] = (uint16_t)(h.col[j] - base[p]);
+        }
+        uint16_t *dc = nullptr;
+        if ((rc = dmalloc(&dc, nnz, acc)) || (rc = dmalloc(&M.panel_base, np, acc)))
+            return rc;
+        M.col = dc;
+        if (nnz)
+            XH_HIP(hipMemcpy(dc, c16.data(), nnz * 2, hipMemcpyHostToDevice));
+        XH_HIP(hipMemcpy(M.panel_base, base.data(), np * 4, hipMemcpyHostToDevice));
+    } else {
+        uint32_t *dc = nullptr;
+        if ((rc = dmalloc(&dc, nnz, acc)))
+            return rc;
+        M.col = dc;
+        if (nnz)
+            XH_HIP(hipMemcpy(dc, h.col.data(), nnz * 4, hipMemcpyHostToDevice));
+    }
+    // values and diagonal inverses in the matrix precision
+    // (csr_matrix::alloc_inv_diag / alloc_inv_sqrt_diag, csr_matrix.h:302-368)
+    if (prec == F64) {
+        std::vector<double> id(n, 0.0), isd(n, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (uint32_t j = h.row[i]; j < h.row[i + 1]; ++j)
+                if (h.col[j] == (uint32_t)i) {
+                    id[i] = 1.0 / h.val[j];
+                    isd[i] = std::sqrt(1.0 / h.val[j]);
+                }
+        double *v, *a, *b;
+        if ((rc = dmalloc(&v, nnz, acc)) || (rc = dmalloc(&a, n, acc)) ||
+            (rc = dmalloc(&b, n, acc)))
+            return rc;
+        if (nnz)
+            XH_HIP(hipMemcpy(v, h.val.data(), nnz * 8, hipMemcpyHostToDevice));
+        XH_HIP(hipMemcpy(a, id.data(), n * 8, hipMemcpyHostToDevice));
+        XH_HIP(hipMemcpy(b, isd.data(), n * 8, hipMemcpyHostToDevice));
+        M.val = v;
+        M.inv_diag = a;
+        M.inv_sqrt_diag = b;
+    } else {
+        std::vector<float> vf(nnz), id(n, 0.f), isd(n, 0.f);
+        for (int64_t j = 0; j < nnz; ++j)
+            vf[j] = (float)h.val[j];
+        for (int64_t i = 0; i < n; ++i)
+            for (uint32_t j = h.row[i]; j < h.row[i + 1]; ++j)
+                if (h.col[j] == (uint32_t)i) {
+                    id[i] = (float)(1.0 / (double)vf[j]);
+                    isd[i] = (float)std::sqrt(1.0 / (double)vf[j]);
+                }
+        float *v, *a, *b;
+        if ((rc = dmalloc(&v, nnz, acc)) || (rc = dmalloc(&a, n, acc)) ||
+            (rc = dmalloc(&b, n, acc)))
+            return rc;
+        if (nnz)
+            XH_HIP(hipMemcpy(v, vf.data(), nnz * 4, hipMemcpyHostToDevice));
+        XH_HIP(hipMemcpy(a, id.data(), n * 4, hipMemcpyHostToDevice));
+        XH_HIP(hipMemcpy(b, isd.data(), n * 4, hipMemcpyHostToDevice));
+        M.val = v;
+        M.inv_diag = a;
+        M.inv_sqrt_diag = b;
+    }
+    M.bytes = acc;
+    return 0;
+}
+
+void free_mat(Mat &M) {
+    for (void *p : {(void *)M.row, M.col, (void *)M.panel_base, M.val, M.inv_diag,
+                    M.inv_sqrt_diag})
+        if (p)
+            (void)hipFree(p);
+    M = Mat();
+}
+
+// Gauss-Jordan without pivoting (src/primitives/matrix/operations/inverse.inl:31-106)
+template <typename T>
+void gj_inverse(int64_t n, const std::vector<T> &A, std::vector<T> &inv) {
+    const int64_t nc = 2 * n;
+    std::vector<T> e((size_t)n * nc, T(0));
+    for (int64_t i = 0; i < n; ++i) {
+        e[i * nc + i + n] = 1.0;
+        for (int64_t j = 0; j < n; ++j)
+            e[i * nc + j] = A[i * n + j];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        T *ri = &e[i * nc];
+        for (int64_t k = nc - 1; k > i; --k)
+            ri[k] /= ri[i];
+        ri[i] = 1.0;
+        for (int64_t j = i + 1; j < n; ++j) {
+            T *rj = &e[j * nc];
+            for (int64_t k = nc - 1; k > i; --k)
+                rj[k] -= rj[i] * ri[k];
+            rj[i] = 0.0;
+        }
+    }
+    for (int64_t ii = 0; ii < n; ++ii) {
+        const int64_t row = n - 1 - ii;
+        T *ri = &e[row * nc];
+        for (int64_t j = 0; j < row; ++j) {
+            T *rj = &e[j * nc];
+            for (int64_t k = nc / 2; k < nc; ++k)
+                rj[k] -= rj[row] * ri[k];
+            rj[row] = 0.0;
+        }
+    }
+    inv.assign((size_t)n * n, T(0));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j)
+            inv[i * n + j] = e[i * nc + n + j];
+}
+
+// Chebyshev_polynomial_coeffs (src/solvers/detail/chebyshev.inl:50-121); orders above 4
+// (the reference asserts) use the same polynomial p(t) = (1 - T_k((th-t)/de)/T_k(th/de))/t
+// expanded by the three-term recurrence
+int cheby_coeffs(double mn, double mx, int order, double fraction, std::vector<double> &c) {
+    const double upper = mx * 1.1;
+    const double lower = (upper - mn) * fraction + mn;
+    const double th = (upper + lower) / 2., de = (upper - lower) / 2.;
+    c.assign(order > 0 ? order : 1, 0.0);
+    double den;
+    switch (order) {
+    case 1:
+        c[0] = 1. / th;
+        return 0;
+    case 2:
+        den = de * de - 2 * th * th;
+        c[0] = -4 * th / den;
+        c[1] = 2 / den;
+        return 0;
+    case 3:
+        den = 3 * (de * de) * th - 4 * (th * th * th);
+        c[0] = (3 * de * de - 12 * th * th) / den;
+        c[1] = 12 * th / den;
+        c[2] = -4 / den;
+        return 0;
+    case 4:
+        den = pow(de, 4) - 8 * de * de * th * th + 8 * pow(th, 4);
+        c[0] = (32 * pow(th, 3) - 16 * de * de * th) / den;
+        c[1] = (8 * de * de - 48 * th * th) / den;
+        c[2] = 32 * th / den;
+        c[3] = -8 / den;
+        return 0;
+    default:
+        break;
+    }
+    if (order < 1 || order > 16)
+        return -1;
+    std::vector<double> tm1(order + 2, 0.0), tm(order + 2, 0.0), tn(order + 2, 0.0);
+    tm1[0] = 1.0;
+    tm[0] = th / de;
+    tm[1] = -1.0 / de;
+    for (int m = 1; m < order; ++m) {
+        std::fill(tn.begin(), tn.end(), 0.0);
+        for (int i = 0; i <= m; ++i) {
+            tn[i] += 2 * (th / de) * tm[i];
+            tn[i + 1] += 2 * (-1.0 / de) * tm[i];
+        }
+        for (int i = 0; i <= m - 1; ++i)
+            tn[i] -= tm1[i];
+        tm1 = tm;
+        tm = tn;
+    }
+    for (int i = 0; i < order; ++i)
+        c[i] = -tm[i + 1] / tm[0];
+    return 0;
+}
+
+} // namespace
+} // namespace xh
+
+using namespace xh;
+
+struct xh_matrix_s {
+    Mat m;
+};
+
+struct xh_hier_s {
+    int nlevels = 0;
+    std::vector<HostCsr> A, P, R;
+    std::vector<double> mn, mx;
+};
+
+namespace xh {
+namespace {
+
+struct Level {
+    int64_t n = 0;
+    int prec = F64, tprec = F64; // operator / transfer precision
+    Mat A, P, R;
+    bool own_A = true;
+    double mn = 0, mx = 0;
+    int64_t hsgs_B = 0;
+    void *tmp = nullptr, *tmp_t = nullptr, *b = nullptr, *x = nullptr;
+    void *sr = nullptr, *su1 = nullptr, *su2 = nullptr;
+    void *dinv = nullptr;
+    int dinv_prec = F64;
+    void *cb64 = nullptr, *cx64 = nullptr; // fp64 coarse solve in an fp32 hierarchy
+    std::vector<double> cheb_pre, cheb_post, cheb_coarse;
+};
+
+} // namespace
+} // namespace xh
+
+struct xh_solver_s {
+    xh_params prm;
+    int nv = 0;
+    int64_t n = 0;
+    std::vector<Level> lv;
+    Mat A0; // fp64 finest operator of the Krylov iteration
+    hipStream_t stream = nullptr;
+    // Krylov vectors (fp64)
+    double *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr;
+    double *p_ = nullptr, *s_ = nullptr, *xi = nullptr, *bi = nullptr;
+    void *b32 = nullptr, *x32 = nullptr; // level-0 fp32 b/x (mg_prec 2)
+    SolverState *st = nullptr;
+    SolverState *hst = nullptr; // pinned host mirror
+    int *hdone = nullptr;       // pinned [2]
+    double *partials = nullptr;
+    double *hist = nullptr;
+    int hist_len = 0;
+    const int *dflag = nullptr; // `done` gate of the launches being enqueued
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const double *g_x = nullptr; // x pointer the graph was captured with
+    size_t bytes = 0;
+    int err = 0;
+};
+
+namespace xh {
+namespace {
+
+template <typename T>
+int alloc_vec(xh_solver_s *S, T **p, size_t count) {
+    return dmalloc(p, count, S->bytes);
+}
+
+int alloc_prec(xh_solver_s *S, void **p, size_t count, int prec) {
+    if (prec == F64)
+        return alloc_vec(S, (double **)p, count);
+    return alloc_vec(S, (float **)p, count);
+}
+
+SpmvArgs args(const xh_solver_s *S) {
+    SpmvArgs a;
+    a.done = S->dflag;
+    return a;
+}
+
+void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<double> &cheb,
+            void *x, const void *b, bool x_zero) {
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    switch (role.method) {
+    case XH_CHEBYSHEV: {
+        const int k = role.poly_order;
+        void *uc = L.su1, *un = L.su2;
+        if (x_zero) {
+            cheb_init_zero(L.n, nv, L.prec, b, L.A.inv_sqrt_diag, cheb[k - 1], L.sr,
+                           k == 1 ? x : uc, k == 1, S->dflag, st);
+        } else {
+            SpmvArgs a = args(S);
+            a.b = b;
+            a.isd = L.A.inv_sqrt_diag;
+            a.out2 = L.sr;
+            a.xupd = x;
+            a.c = cheb[k - 1];
+            a.last = k == 1;
+            a.x_zero = 0;
+            spmv(L.A, nv, x, uc, OP_CHEB0, a, st);
+        }
+        for (int i = k - 2; i >= 0; --i) {
+            SpmvArgs a = args(S);
+            a.isd = L.A.inv_sqrt_diag;
+            a.r = L.sr;
+            a.c = cheb[i];
+            a.last = i == 0;
+            a.xupd = x;
+            a.x_zero = x_zero ? 1 : 0;
+            spmv(L.A, nv, uc, un, OP_CHEB, a, st);
+            std::swap(uc, un);
+        }
+        return;
+    }
+    case XH_JACOBI: {
+        const double relax = L.prec == F64 ? (double)role.relax_factor
+                                           : (double)(float)role.relax_factor;
+        for (int it = 0; it < std::max(1, role.max_iters); ++it) {
+            if (x_zero) {
+                jacobi_update(L.n, nv, L.prec, x, b, nullptr, L.A.inv_diag, relax, 1, S->dflag,
+                              st);
+            } else {
+                spmv(L.A, nv, x, L.sr, OP_SET, args(S), st);
+                jacobi_update(L.n, nv, L.prec, x, b, L.sr, L.A.inv_diag, relax, 0, S->dflag,
+                              st);
+            }
+            x_zero = false;
+        }
+        return;
+    }
+    case XH_HSGS: {
+        const double relax = (double)role.relax_factor;
+        for (int it = 0; it < std::max(1, role.max_iters); ++it) {
+            if (x_zero) {
+                hsgs_sweeps(L.A, nv, b, x, 1, relax, L.hsgs_B, S->dflag, st);
+            } else {
+                SpmvArgs a = args(S);
+                a.b = b;
+                a.blk_rows = L.hsgs_B;
+                spmv(L.A, nv, x, L.sr, OP_OFFBLOCK, a, st);
+                hsgs_sweeps(L.A, nv, L.sr, x, 0, relax, L.hsgs_B, S->dflag, st);
+            }
+            x_zero = false;
+        }
+        return;
+    }
+    case XH_DIRECT: {
+        if (L.cb64) {
+            convert(L.n * nv, F32, b, L.cb64, S->dflag, st);
+            dense_matvec(L.n, nv, F64, L.dinv, L.cb64, L.cx64, S->dflag, st);
+            convert(L.n * nv, F64, L.cx64, x, S->dflag, st);
+        } else {
+            dense_matvec(L.n, nv, L.dinv_prec, L.dinv, b, x, S->dflag, st);
+        }
+        return;
+    }
+    case XH_IDENTITY:
+    default:
+        copy_vec(L.n * nv, L.prec, b, x, S->dflag, st);
+        return;
+    }
+}
+
+// MultiGrid::MG_cycle (mg.inl:166-260); x_zero mirrors the reference's if_zero flag
+void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cyc) {
+    Level &L = S->lv[l];
+    Level &H = S->lv[l + 1];
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    const int nl = (int)S->lv.size();
+    if (cyc != 2) {
+        smooth(S, S->prm.pre, L, L.cheb_pre, x, b, x_zero);
+        x_zero = false;
+    }
+    void *rt = (L.tprec == L.prec) ? L.tmp : L.tmp_t;
+    if (x_zero) {
+        if (L.tprec == L.prec)
+            copy_vec(L.n * nv, L.prec, b, rt, S->dflag, st);
+        else
+            convert(L.n * nv, L.prec, b, rt, S->dflag, st);
+    } else {
+        SpmvArgs a = args(S);
+        a.b = b;
+        a.out_f32 = (L.tprec != L.prec) ? 1 : 0;
+        spmv(L.A, nv, x, rt, OP_RESID, a, st);
+    }
+    spmv(L.R, nv, rt, H.b, OP_SET, args(S), st);
+    if (l + 1 == nl - 1) {
+        smooth(S, S->prm.coarse, H, H.cheb_coarse, H.x, H.b, true);
+    } else {
+        mg_cycle(S, l + 1, H.x, H.b, true, cyc);
+        if (cyc == 1)
+            mg_cycle(S, l + 1, H.x, H.b, false, 1);
+        else if (cyc == 2)
+            mg_cycle(S, l + 1, H.x, H.b, false, 0);
+    }
+    if (L.tprec == L.prec) {
+        spmv(L.P, nv, H.x, x, x_zero ? OP_SET : OP_ADD, args(S), st);
+    } else {
+        SpmvArgs a = args(S);
+        a.x_zero = x_zero ? 1 : 0;
+        spmv(L.P, nv, H.x, x, OP_PROLONG32, a, st);
+    }
+    smooth(S, S->prm.post, L, L.cheb_post, x, b, false);
+}
+
+// preconditioner application x = M^{-1} b, x zero-flagged on entry
+void precond(xh_solver_s *S, const double *b, double *x) {
+    const xh_role &pr = S->prm.precond;
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    Level &L0 = S->lv[0];
+    if (pr.method == XH_MULTIGRID) {
+        const int nl = (int)S->lv.size();
+        const int iters = std::max(1, pr.max_iters);
+        if (S->prm.mg_prec == 2) {
+            convert(S->n * nv, F64, b, S->b32, S->dflag, st);
+            for (int it = 0; it < iters; ++it) {
+                if (nl == 1)
+                    smooth(S, S->prm.coarse, L0, L0.cheb_coarse, S->x32, S->b32, it == 0);
+                else
+                    mg_cycle(S, 0, S->x32, S->b32, it == 0, S->prm.mg_cycle);
+            }
+            convert(S->n * nv, F32, S->x32, x, S->dflag, st);
+        } else {
+            for (int it = 0; it < iters; ++it) {
+                if (nl == 1)
+                    smooth(S, S->prm.coarse, L0, L0.cheb_coarse, x, b, it == 0);
+                else
+                    mg_cycle(S, 0, x, b, it == 0, S->prm.mg_cycle);
+            }
+        }
+        return;
+    }
+    smooth(S, pr, L0, L0.cheb_pre, x, b, true);
+}
+
+bool is_preconditioned(int m) { return m == XH_PBICGSTAB || m == XH_MERGED_PBICGSTAB; }
+
+void enqueue_init(xh_solver_s *S, const double *b, double *x) {
+    SpmvArgs a = args(S);
+    a.b = b;
+    a.out2 = S->p;
+    a.out3 = S->r;
+    a.partials = S->partials;
+    int np = spmv(S->A0, S->nv, x, S->r0, OP_RESID0, a, S->stream);
+    finalize(FIN_INIT, S->st, S->partials, np, S->nv, S->stream);
+}
+
+void enqueue_iteration(xh_solver_s *S, double *x) {
+    const int m = S->prm.solver.method;
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    const bool pre = is_preconditioned(m);
+    double *pp = pre ? S->p_ : S->p;
+    double *ss = pre ? S->s_ : S->s;
+    if (pre)
+        precond(S, S->p, S->p_);
+    SpmvArgs a = args(S);
+    a.o1 = S->r0;
+    a.partials = S->partials;
+    int np = spmv(S->A0, nv, pp, S->v, OP_DOT1, a, st);
+    finalize(FIN_ALPHA, S->st, S->partials, np, nv, st);
+    VecArgs va;
+    va.st = S->st;
+    va.s = S->s;
+    va.r = S->r;
+    va.v = S->v;
+    vec_op(V_S_UPDATE, S->n, nv, va, st);
+    if (pre)
+        precond(S, S->s, S->s_);
+    SpmvArgs a3 = args(S);
+    a3.o1 = S->s;
+    a3.partials = S->partials;
+    np = spmv(S->A0, nv, ss, S->t, OP_DOT3, a3, st);
+    if (m == XH_MERGED_BICGSTAB) {
+        finalize(FIN_OMEGA_MERGED, S->st, S->partials, np, nv, st);
+        VecArgs vx;
+        vx.st = S->st;
+        vx.x = x;
+        vx.p_ = S->p;
+        vx.s_ = S->s;
+        vx.gate = 1;
+        vec_op(V_X_UPDATE, S->n, nv, vx, st);
+        VecArgs vr;
+        vr.st = S->st;
+        vr.s = S->s;
+        vr.t = S->t;
+        vr.r = S->r;
+        vr.r0 = S->r0;
+        vr.partials = S->partials;
+        np = vec_op(V_R_UPDATE, S->n, nv, vr, st);
+        finalize(FIN_BETA, S->st, S->partials, np, nv, st);
+        VecArgs v5;
+        v5.st = S->st;
+        v5.x = x;
+        v5.p = S->p;
+        v5.s = S->s;
+        v5.r = S->r;
+        v5.v = S->v;
+        vec_op(V_GROUP5, S->n, nv, v5, st);
+        return;
+    }
+    finalize(FIN_OMEGA, S->st, S->partials, np, nv, st);
+    VecArgs vx;
+    vx.st = S->st;
+    vx.x = x;
+    vx.p_ = pp;
+    vx.s_ = ss;
+    vx.s = S->s;
+    vx.t = S->t;
+    vx.r = S->r;
+    vx.r0 = S->r0;
+    vx.partials = S->partials;
+    np = vec_op(V_XR_UPDATE, S->n, nv, vx, st);
+    finalize(FIN_BETA, S->st, S->partials, np, nv, st);
+    VecArgs vp;
+    vp.st = S->st;
+    vp.p = S->p;
+    vp.r = S->r;
+    vp.v = S->v;
+    vec_op(V_P_UPDATE, S->n, nv, vp, st);
+}
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(XH_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int solve_impl(xh_solver_s *S, const double *db, double *dx) {
+    const int nv = S->nv;
+    // initial state (solver_stats::reset + iconv = ones, solver.h:54-65)
+    SolverState &h = *S->hst;
+    std::memset(&h, 0, sizeof(SolverState));
+    for (int k = 0; k < nv; ++k)
+        h.iconv[k] = 1.0;
+    h.nv = nv;
+    h.max_iters = S->prm.solver.max_iters;
+    h.conv_check = S->prm.solver.conv_check;
+    h.abs_tol = S->prm.solver.abs_tol;
+    h.rel_tol = S->prm.solver.rel_tol;
+    h.hist = S->hist;
+    h.hist_len = S->hist_len;
+    XH_HIP(hipMemcpyAsync(S->st, S->hst, sizeof(SolverState), hipMemcpyHostToDevice, S->stream));
+    S->dflag = &S->st->done;
+    enqueue_init(S, db, dx);
+    int rc = check_launch();
+    if (rc)
+        return rc;
+    const bool use_graph = S->prm.graph != 0;
+    if (use_graph && (S->gexec == nullptr || S->g_x != dx)) {
+        if (S->gexec) {
+            (void)hipGraphExecDestroy(S->gexec);
+            (void)hipGraphDestroy(S->graph);
+            S->gexec = nullptr;
+            S->graph = nullptr;
+        }
+        XH_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(S, dx);
+        XH_HIP(hipStreamEndCapture(S->stream, &S->graph));
+        XH_HIP(hipGraphInstantiate(&S->gexec, S->graph, nullptr, nullptr, 0));
+        S->g_x = dx;
+    }
+    const int maxit = std::max(1, S->prm.solver.max_iters) + 1;
+    for (int it = 0; it < maxit; ++it) {
+        if (use_graph) {
+            XH_HIP(hipGraphLaunch(S->gexec, S->stream));
+        } else {
+            enqueue_iteration(S, dx);
+            if ((rc = check_launch()))
+                return rc;
+        }
+        XH_HIP(hipMemcpyAsync(&S->hdone[it & 1], &S->st->done, sizeof(int),
+                              hipMemcpyDeviceToHost, S->stream));
+        XH_HIP(hipEventRecord(S->ev[it & 1], S->stream));
+        if (it > 0) {
+            XH_HIP(hipEventSynchronize(S->ev[(it - 1) & 1]));
+            if (S->hdone[(it - 1) & 1])
+                break;
+        }
+    }
+    XH_HIP(hipMemcpyAsync(S->hst, S->st, sizeof(SolverState), hipMemcpyDeviceToHost, S->stream));
+    XH_HIP(hipStreamSynchronize(S->stream));
+    S->dflag = nullptr;
+    if (!S->hst->done)
+        return fail(XH_EINVAL, "solve did not terminate within max_iters");
+    return 0;
+}
+
+} // namespace
+} // namespace xh
+
+extern "C" {
+
+int xh_init(int device) {
+    int n = 0;
+    XH_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(XH_EINVAL, "no such HIP device");
+    XH_HIP(hipSetDevice(device));
+    if (g_stream == nullptr)
+        XH_HIP(hipStreamCreate(&g_stream));
+    g_device = device;
+    return 0;
+}
+
+int xh_finalize(void) {
+    if (g_stream) {
+        (void)hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+    }
+    g_device = -1;
+    return 0;
+}
+
+const char *xh_last_error(void) { return g_err.c_str(); }
+
+int xh_device_info(char *name, int name_len, int *compute_units, double *hbm_bytes) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    hipDeviceProp_t p;
+    XH_HIP(hipGetDeviceProperties(&p, g_device));
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (compute_units)
+        *compute_units = p.multiProcessorCount;
+    if (hbm_bytes)
+        *hbm_bytes = (double)p.totalGlobalMem;
+    return 0;
+}
+
+int xh_malloc(void **dptr, size_t bytes) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    XH_HIP(hipMalloc(dptr, bytes));
+    return 0;
+}
+
+int xh_free(void *dptr) {
+    XH_HIP(hipFree(dptr));
+    return 0;
+}
+
+int xh_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    XH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int xh_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    XH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int xh_synchronize(void) {
+    XH_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+int xh_matrix_create(int64_t nrows, int64_t ncols, int64_t nnz, const uint32_t *row,
+                     const uint32_t *col, const double *val, int prec, int index_mode,
+                     xh_matrix *out) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (!out || !row || (nnz > 0 && (!col || !val)) || nrows < 0 || ncols < 0)
+        return fail(XH_EINVAL, "xh_matrix_create: bad arguments");
+    if (prec != F64 && prec != F32)
+        return fail(XH_EINVAL, "xh_matrix_create: prec must be 0 (f64) or 1 (f32)");
+    if ((int64_t)row[nrows] != nnz || row[0] != 0)
+        return fail(XH_EINVAL, "xh_matrix_create: row pointer does not match nnz");
+    HostCsr h;
+    h.nrows = nrows;
+    h.ncols = ncols;
+    h.row.assign(row, row + nrows + 1);
+    h.col.assign(col, col + nnz);
+    h.val.assign(val, val + nnz);
+    for (int64_t i = 0; i < nrows; ++i)
+        if (h.row[i + 1] < h.row[i])
+            return fail(XH_EINVAL, "xh_matrix_create: row pointer not monotone");
+    for (int64_t j = 0; j < nnz; ++j)
+        if ((int64_t)h.col[j] >= ncols)
+            return fail(XH_EINVAL, "xh_matrix_create: column index out of range");
+    sort_diag_first(h);
+    auto *m = new xh_matrix_s();
+    rc = upload_mat(h, prec, index_mode, m->m);
+    if (rc) {
+        free_mat(m->m);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+int xh_matrix_info(xh_matrix A, int64_t *nrows, int64_t *ncols, int64_t *nnz, int *prec,
+                   int *index_bits, int64_t *panel_rows, size_t *device_bytes) {
+    if (!A)
+        return fail(XH_EINVAL, "null matrix");
+    if (nrows)
+        *nrows = A->m.nrows;
+    if (ncols)
+        *ncols = A->m.ncols;
+    if (nnz)
+        *nnz = A->m.nnz;
+    if (prec)
+        *prec = A->m.prec;
+    if (index_bits)
+        *index_bits = A->m.idx == IDX_U16 ? 16 : 32;
+    if (panel_rows)
+        *panel_rows = A->m.idx == IDX_U16 ? ((int64_t)1 << A->m.panel_shift) : 0;
+    if (device_bytes)
+        *device_bytes = A->m.bytes;
+    return 0;
+}
+
+int xh_matrix_destroy(xh_matrix A) {
+    if (A) {
+        free_mat(A->m);
+        delete A;
+    }
+    return 0;
+}
+
+int xh_spmv(xh_matrix A, int nv, const void *dx, void *dy, int accumulate) {
+    if (!A || !nv_ok(nv) || !dx || !dy)
+        return fail(XH_EINVAL, "xh_spmv: bad arguments");
+    SpmvArgs a;
+    if (spmv(A->m, nv, dx, dy, accumulate ? OP_ADD : OP_SET, a, g_stream) < 0)
+        return fail(XH_ENOTSUP, "xh_spmv: unsupported configuration");
+    XH_HIP(hipGetLastError());
+    XH_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int xh_spmv_timed(xh_matrix A, int nv, const void *dx, void *dy, int accumulate, int reps,
+                  double *ms_per_launch) {
+    if (!A || !nv_ok(nv) || !dx || !dy || reps < 1)
+        return fail(XH_EINVAL, "xh_spmv_timed: bad arguments");
+    hipEvent_t e0, e1;
+    XH_HIP(hipEventCreate(&e0));
+    XH_HIP(hipEventCreate(&e1));
+    SpmvArgs a;
+    XH_HIP(hipEventRecord(e0, g_stream));
+    for (int i = 0; i < reps; ++i)
+        spmv(A->m, nv, dx, dy, accumulate ? OP_ADD : OP_SET, a, g_stream);
+    XH_HIP(hipEventRecord(e1, g_stream));
+    XH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    XH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_launch)
+        *ms_per_launch = (double)ms / reps;
+    XH_HIP(hipGetLastError());
+    return 0;
+}
+
+int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy) {
+    if (n <= 0 || !nv_ok(nv) || !dD || !dx || !dy)
+        return fail(XH_EINVAL, "xh_dense_matvec: bad arguments");
+    dense_matvec(n, nv, prec, dD, dx, dy, nullptr, g_stream);
+    XH_HIP(hipGetLastError());
+    XH_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int xh_hier_create(int nlevels, xh_hier *out) {
+    if (nlevels < 1 || !out)
+        return fail(XH_EINVAL, "xh_hier_create: nlevels must be >= 1");
+    auto *H = new xh_hier_s();
+    H->nlevels = nlevels;
+    H->A.resize(nlevels);
+    H->P.resize(nlevels);
+    H->R.resize(nlevels);
+    H->mn.assign(nlevels, 0.0);
+    H->mx.assign(nlevels, 0.0);
+    *out = H;
+    return 0;
+}
+
+static int copy_csr(HostCsr &d, int64_t nr, int64_t nc, int64_t nnz, const uint32_t *row,
+                    const uint32_t *col, const double *val) {
+    if (!row || (nnz > 0 && (!col || !val)))
+        return fail(XH_EINVAL, "xh_hier_set_level: missing CSR array");
+    if ((int64_t)row[nr] != nnz)
+        return fail(XH_EINVAL, "xh_hier_set_level: row pointer does not match nnz");
+    d.nrows = nr;
+    d.ncols = nc;
+    d.row.assign(row, row + nr + 1);
+    d.col.assign(col, col + nnz);
+    d.val.assign(val, val + nnz);
+    for (int64_t j = 0; j < nnz; ++j)
+        if ((int64_t)d.col[j] >= nc)
+            return fail(XH_EINVAL, "xh_hier_set_level: column index out of range");
+    return 0;
+}
+
+int xh_hier_set_level(xh_hier H, int level, int64_t n, int64_t nnzA, const uint32_t *rowA,
+                      const uint32_t *colA, const double *valA, int64_t nc, int64_t nnzP,
+                      const uint32_t *rowP, const uint32_t *colP, const double *valP,
+                      int64_t nnzR, const uint32_t *rowR, const uint32_t *colR,
+                      const double *valR, double min_eig, double max_eig) {
+    if (!H || level < 0 || level >= H->nlevels)
+        return fail(XH_EINVAL, "xh_hier_set_level: bad level");
+    int rc = copy_csr(H->A[level], n, n, nnzA, rowA, colA, valA);
+    if (rc)
+        return rc;
+    sort_diag_first(H->A[level]);
+    if (level + 1 < H->nlevels) {
+        if ((rc = copy_csr(H->P[level], n, nc, nnzP, rowP, colP, valP)))
+            return rc;
+        if ((rc = copy_csr(H->R[level], nc, n, nnzR, rowR, colR, valR)))
+            return rc;
+    }
+    H->mn[level] = min_eig;
+    H->mx[level] = max_eig;
+    return 0;
+}
+
+int xh_hier_destroy(xh_hier H) {
+    delete H;
+    return 0;
+}
+
+int xh_solver_destroy(xh_solver S);
+
+int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (!H || !prm || !out || !nv_ok(nv))
+        return fail(XH_EINVAL, "xh_solver_create: bad arguments (nv must be 1,2,4,...,64)");
+    const int m = prm->solver.method;
+    if (m != XH_BICGSTAB && m != XH_PBICGSTAB && m != XH_MERGED_BICGSTAB &&
+        m != XH_MERGED_PBICGSTAB)
+        return fail(XH_ENOTSUP, "xh_solver_create: solver method must be a BiCGStab variant");
+    if (is_preconditioned(m) && prm->precond.method == XH_NONE)
+        return fail(XH_EINVAL, "xh_solver_create: preconditioned method without preconditioner");
+    const int nl = H->nlevels;
+    for (int l = 0; l < nl; ++l)
+        if (H->A[l].nrows == 0 || (l + 1 < nl && H->P[l].ncols != H->A[l + 1].nrows))
+            return fail(XH_EINVAL, "xh_solver_create: inconsistent hierarchy");
+    auto check_role = [](const xh_role &r, bool smoother) {
+        if (r.method == XH_CHEBYSHEV && (r.poly_order < 1 || r.poly_order > 16))
+            return false;
+        if (smoother && !(r.method == XH_JACOBI || r.method == XH_HSGS ||
+                          r.method == XH_CHEBYSHEV || r.method == XH_DIRECT ||
+                          r.method == XH_IDENTITY))
+            return false;
+        return true;
+    };
+    const bool mg = is_preconditioned(m) && prm->precond.method == XH_MULTIGRID;
+    if (mg && (!check_role(prm->pre, true) || !check_role(prm->post, true) ||
+               !check_role(prm->coarse, true)))
+        return fail(XH_ENOTSUP, "xh_solver_create: unsupported smoother / coarse solver");
+    if (is_preconditioned(m) && !mg && !check_role(prm->precond, true))
+        return fail(XH_ENOTSUP, "xh_solver_create: unsupported preconditioner");
+    if (prm->mg_prec < 0 || prm->mg_prec > 2)
+        return fail(XH_EINVAL, "xh_solver_create: mg_prec must be 0, 1 or 2");
+
+    auto *S = new xh_solver_s();
+    S->prm = *prm;
+    S->nv = nv;
+    S->n = H->A[0].nrows;
+    XH_HIP(hipStreamCreate(&S->stream));
+    XH_HIP(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
+    XH_HIP(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
+    auto bail = [&](int code) {
+        xh_solver_destroy(S);
+        return code;
+    };
+    // finest fp64 operator of the Krylov loop
+    if ((rc = upload_mat(H->A[0], F64, prm->index_mode, S->A0)))
+        return bail(rc);
+    S->bytes += S->A0.bytes;
+    const int nlev = mg ? nl : 1;
+    S->lv.resize(nlev);
+    const int mp = prm->mg_prec;
+    for (int l = 0; l < nlev; ++l) {
+        Level &L = S->lv[l];
+        L.n = H->A[l].nrows;
+        L.prec = (mg && (mp == 2 || (mp == 1 && l >= 1))) ? F32 : F64;
+        L.tprec = (mg && mp != 0) ? F32 : F64;
+        L.mn = H->mn[l];
+        L.mx = H->mx[l];
+        if (l == 0 && L.prec == F64) {
+            L.A = S->A0;
+            L.own_A = false;
+        } else {
+            if ((rc = upload_mat(H->A[l], L.prec, l == 0 ? prm->index_mode : 0, L.A)))
+                return bail(rc);
+            S->bytes += L.A.bytes;
+        }
+        if (mg && l + 1 < nlev) {
+            if ((rc = upload_mat(H->P[l], L.tprec, 0, L.P)) ||
+                (rc = upload_mat(H->R[l], L.tprec, 0, L.R)))
+                return bail(rc);
+            S->bytes += L.P.bytes + L.R.bytes;
+        }
+        int nb = std::max(1, prm->hsgs_ncores);
+        if (prm->hsgs_block_rows > 0)
+            nb = (int)std::max<int64_t>(1, (L.n + prm->hsgs_block_rows - 1) / prm->hsgs_block_rows);
+        L.hsgs_B = hsgs_block_rows(L.n, nb);
+        const size_t cnt = (size_t)L.n * nv;
+        if ((rc = alloc_prec(S, &L.sr, cnt, L.prec)) || (rc = alloc_prec(S, &L.su1, cnt, L.prec)) ||
+            (rc = alloc_prec(S, &L.su2, cnt, L.prec)) || (rc = alloc_prec(S, &L.tmp, cnt, L.prec)))
+            return bail(rc);
+        if (L.tprec != L.prec && (rc = alloc_prec(S, &L.tmp_t, cnt, L.tprec)))
+            return bail(rc);
+        if (l >= 1 && ((rc = alloc_prec(S, &L.b, cnt, L.prec)) ||
+                       (rc = alloc_prec(S, &L.x, cnt, L.prec))))
+            return bail(rc);
+        // Chebyshev coefficients per role (the smoother objects of MultiGrid::setup,
+        // mg.inl:113-131, each evaluate them from the level's eigenvalue estimates)
+        const xh_role *roles[3] = {mg ? &prm->pre : &prm->precond, &prm->post, &prm->coarse};
+        std::vector<double> *dst[3] = {&L.cheb_pre, &L.cheb_post, &L.cheb_coarse};
+        for (int q = 0; q < 3; ++q)
+            if (roles[q]->method == XH_CHEBYSHEV &&
+                cheby_coeffs(L.mn, L.mx, roles[q]->poly_order, (double)roles[q]->spectrum_fraction,
+                             *dst[q]))
+                return bail(fail(XH_EINVAL, "Chebyshev order out of range"));
+    }
+    // coarsest direct solve: dense Gauss-Jordan inverse on the host (Direct::init)
+    {
+        Level &C = S->lv[nlev - 1];
+        const bool need = (mg && prm->coarse.method == XH_DIRECT) ||
+                          (!mg && prm->precond.method == XH_DIRECT);
+        if (need) {
+            const HostCsr &Ac = H->A[nlev - 1];
+            const int64_t nc = Ac.nrows;
+            if (nc > 8192)
+                return bail(fail(XH_EINVAL, "Direct coarse solve on a level larger than 8192"));
+            C.dinv_prec = C.prec;
+            if (mg && mp == 2 && prm->coarse_fp64) {
+                C.dinv_prec = F64;
+                if ((rc = alloc_vec(S, (double **)&C.cb64, (size_t)nc * nv)) ||
+                    (rc = alloc_vec(S, (double **)&C.cx64, (size_t)nc * nv)))
+                    return bail(rc);
+            }
+            if (C.dinv_prec == F64) {
+                std::vector<double> dense((size_t)nc * nc, 0.0), inv;
+                for (int64_t i = 0; i < nc; ++i)
+                    for (uint32_t j = Ac.row[i]; j < Ac.row[i + 1]; ++j)
+                        dense[i * nc + Ac.col[j]] = Ac.val[j];
+                gj_inverse<double>(nc, dense, inv);
+                if ((rc = alloc_vec(S, (double **)&C.dinv, (size_t)nc * nc)))
+                    return bail(rc);
+                XH_HIP(hipMemcpy(C.dinv, inv.data(), inv.size() * 8, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> dense((size_t)nc * nc, 0.f), inv;
+                for (int64_t i = 0; i < nc; ++i)
+                    for (uint32_t j = Ac.row[i]; j < Ac.row[i + 1]; ++j)
+                        dense[i * nc + Ac.col[j]] = (float)Ac.val[j];
+                gj_inverse<float>(nc, dense, inv);
+                if ((rc = alloc_vec(S, (float **)&C.dinv, (size_t)nc * nc)))
+                    return bail(rc);
+                XH_HIP(hipMemcpy(C.dinv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+            }
+        }
+    }
+    const size_t cnt = (size_t)S->n * nv;
+    for (double **q : {&S->r0, &S->r, &S->p, &S->v, &S->s, &S->t, &S->p_, &S->s_, &S->xi, &S->bi})
+        if ((rc = alloc_vec(S, q, cnt)))
+            return bail(rc);
+    if (mg && mp == 2) {
+        if ((rc = alloc_prec(S, &S->b32, cnt, F32)) || (rc = alloc_prec(S, &S->x32, cnt, F32)))
+            return bail(rc);
+    }
+    if ((rc = alloc_vec(S, &S->st, 1)) ||
+        (rc = alloc_vec(S, &S->partials, (size_t)kMaxPartials * 3 * kMaxNV)))
+        return bail(rc);
+    S->hist_len = std::max(1, prm->solver.max_iters) + 2;
+    if ((rc = alloc_vec(S, &S->hist, (size_t)S->hist_len * nv)))
+        return bail(rc);
+    XH_HIP(hipHostMalloc((void **)&S->hst, sizeof(SolverState), hipHostMallocDefault));
+    XH_HIP(hipHostMalloc((void **)&S->hdone, 2 * sizeof(int), hipHostMallocDefault));
+    *out = S;
+    return 0;
+}
+
+int xh_solver_solve_device(xh_solver S, const double *db, double *dx) {
+    if (!S || !db || !dx)
+        return fail(XH_EINVAL, "xh_solver_solve_device: bad arguments");
+    return solve_impl(S, db, dx);
+}
+
+int xh_solver_solve(xh_solver S, const double *b, double *x) {
+    if (!S || !b || !x)
+        return fail(XH_EINVAL, "xh_solver_solve: bad arguments");
+    const size_t bytes = (size_t)S->n * S->nv * sizeof(double);
+    XH_HIP(hipMemcpy(S->bi, b, bytes, hipMemcpyHostToDevice));
+    XH_HIP(hipMemcpy(S->xi, x, bytes, hipMemcpyHostToDevice));
+    int rc = solve_impl(S, S->bi, S->xi);
+    if (rc)
+        return rc;
+    XH_HIP(hipMemcpy(x, S->xi, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int xh_solver_stats(xh_solver S, int32_t *iters, int32_t *converged, double *abs_res,
+                    double *rel_res) {
+    if (!S)
+        return fail(XH_EINVAL, "null solver");
+    const SolverState &h = *S->hst;
+    if (iters)
+        *iters = h.iters;
+    for (int k = 0; k < S->nv; ++k) {
+        if (converged)
+            converged[k] = h.conv[k];
+        if (abs_res)
+            abs_res[k] = h.abs_res[k];
+        if (rel_res)
+            rel_res[k] = h.rel_res[k];
+    }
+    return 0;
+}
+
+int xh_solver_history(xh_solver S, double *hist, int32_t max_checks, int32_t *nchecks) {
+    if (!S || !hist)
+        return fail(XH_EINVAL, "xh_solver_history: bad arguments");
+    const int checks = std::min<int>(S->hst->iters + 1, std::min(max_checks, S->hist_len));
+    XH_HIP(hipMemcpy(hist, S->hist, (size_t)checks * S->nv * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    if (nchecks)
+        *nchecks = checks;
+    return 0;
+}
+
+int xh_solver_precond_apply(xh_solver S, const double *db, double *dx) {
+    if (!S || !db || !dx)
+        return fail(XH_EINVAL, "xh_solver_precond_apply: bad arguments");
+    if (!is_preconditioned(S->prm.solver.method))
+        return fail(XH_EINVAL, "xh_solver_precond_apply: solver has no preconditioner");
+    S->dflag = nullptr;
+    precond(S, db, dx);
+    int rc = check_launch();
+    if (rc)
+        return rc;
+    XH_HIP(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+int xh_solver_renew(xh_solver S, const xh_role *solver) {
+    if (!S || !solver)
+        return fail(XH_EINVAL, "xh_solver_renew: bad arguments");
+    if (solver->method != S->prm.solver.method)
+        return fail(XH_EINVAL, "xh_solver_renew: the solver method cannot change");
+    const int need = std::max(1, solver->max_iters) + 2;
+    if (need > S->hist_len) {
+        double *h = nullptr;
+        XH_HIP(hipMalloc((void **)&h, (size_t)need * S->nv * sizeof(double)));
+        (void)hipFree(S->hist);
+        S->hist = h;
+        S->hist_len = need;
+    }
+    S->prm.solver = *solver;
+    return 0;
+}
+
+int xh_solver_info(xh_solver S, int *nlevels, size_t *device_bytes) {
+    if (!S)
+        return fail(XH_EINVAL, "null solver");
+    if (nlevels)
+        *nlevels = (int)S->lv.size();
+    if (device_bytes)
+        *device_bytes = S->bytes;
+    return 0;
+}
+
+int xh_solver_destroy(xh_solver S) {
+    if (!S)
+        return 0;
+    if (S->stream)
+        (void)hipStreamSynchronize(S->stream);
+    if (S->gexec)
+        (void)hipGraphExecDestroy(S->gexec);
+    if (S->graph)
+        (void)hipGraphDestroy(S->graph);
+    for (Level &L : S->lv) {
+        if (L.own_A)
+            free_mat(L.A);
+        free_mat(L.P);
+        free_mat(L.R);
+        for (void *q : {L.tmp, L.tmp_t, L.b, L.x, L.sr, L.su1, L.su2, L.dinv, L.cb64, L.cx64})
+            if (q)
+                (void)hipFree(q);
+    }
+    free_mat(S->A0);
+    for (void *q : {(void *)S->r0, (void *)S->r, (void *)S->p, (void *)S->v, (void *)S->s,
+                    (void *)S->t, (void *)S->p_, (void *)S->s_, (void *)S->xi, (void *)S->bi,
+                    S->b32, S->x32, (void *)S->st, (void *)S->partials, (void *)S->hist})
+        if (q)
+            (void)hipFree(q);
+    if (S->hst)
+        (void)hipHostFree(S->hst);
+    if (S->hdone)
+        (void)hipHostFree(S->hdone);
+    for (hipEvent_t e : S->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (S->stream)
+        (void)hipStreamDestroy(S->stream);
+    delete S;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,649 @@
+// Level-1 / merged BiCGStab vector kernels with fused per-RHS dot partials, the
+// single-workgroup finalize kernels that turn partials into the Krylov scalars and the
+// convergence decision on the device, and the smoother / coarse-solve kernels.
+//
+// References: blas::axpby_z / axpbypcz / xdivy_z (src/blas/detail/blas_ext.inl:31-239),
+// merged_bicgstab_group1/2/4/5 (src/blas/detail/specific/blas_merged_bicgstab.inl:33-257),
+// base_solver::convergence_check / converged (src/solvers/solver.h:173-241),
+// merged_jacobi (src/blas/detail/specific/blas_merged_jacobi.inl:33-90),
+// csr_SGS (src/blas2/detail/csr_gs.inl:33-134), dense_Ax_y
+// (src/blas2/detail/dense_matvec.inl:72-121), vector::convert
+// (src/primitives/vector/operations/convert.inl:31-83).
+//
+// Every expression keeps the reference's operand order; the file is built with
+// -ffp-contract=off so a*b+c rounds twice as in the reference's x86-64 build.
+#include "xh_internal.hpp"
+
+#include <algorithm>
+
+namespace xh {
+namespace {
+
+__device__ __forceinline__ int64_t logical_block() {
+    const uint32_t G = gridDim.x, p = blockIdx.x;
+    return (int64_t)(p & 7u) * (int64_t)(G >> 3) + (int64_t)(p >> 3);
+}
+
+template <int NV, int ND>
+__device__ __forceinline__ void block_partials(double (&p)[ND], double *__restrict__ out,
+                                               int64_t lb) {
+    __shared__ double red[kBlock / 64][ND][NV];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int off = NV; off < 64; off <<= 1)
+            p[d] = p[d] + __shfl_xor(p[d], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < NV)
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+            red[w][d][lane] = p[d];
+    __syncthreads();
+    if ((int)threadIdx.x < ND * NV) {
+        const int d = threadIdx.x / NV, k = threadIdx.x % NV;
+        double s = red[0][d][k];
+#pragma unroll
+        for (int q = 1; q < kBlock / 64; ++q)
+            s = s + red[q][d][k];
+        out[lb * (ND * NV) + d * NV + k] = s;
+    }
+}
+
+enum Gate : int { GATE_NOT_DONE = 0, GATE_CONV_ALL = 1 };
+
+template <int NV, int OP>
+__global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double *__restrict__ x,
+                                                double *__restrict__ p, double *__restrict__ s,
+                                                double *__restrict__ r,
+                                                const double *__restrict__ r0,
+                                                const double *__restrict__ v,
+                                                const double *__restrict__ t,
+                                                const double *__restrict__ p_,
+                                                const double *__restrict__ s_,
+                                                const SolverState *__restrict__ st, int gate,
+                                                double *__restrict__ partials) {
+    if (st->done)
+        return;
+    if (gate == GATE_CONV_ALL && !st->conv_all)
+        return;
+    constexpr int ND = (OP == V_XR_UPDATE || OP == V_R_UPDATE || OP == V_DOT1) ? 1
+                       : (OP == V_DOT3)                                         ? 3
+                                                                                : 0;
+    constexpr int RPB = kBlock / NV;
+    const int64_t lb = logical_block();
+    const int64_t start = lb * chunk;
+    const int64_t end = start + chunk < n ? start + chunk : n;
+    const int tr = threadIdx.x / NV, k = threadIdx.x % NV;
+    double part[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int d = 0; d < (ND > 0 ? ND : 1); ++d)
+        part[d] = 0.0;
+    double c1 = 0, c2 = 0, c3 = 0;
+    if constexpr (OP == V_S_UPDATE) {
+        c1 = st->nalpha[k];
+    } else if constexpr (OP == V_XR_UPDATE) {
+        c1 = st->ac[k];
+        c2 = st->oc[k];
+        c3 = st->omega[k];
+    } else if constexpr (OP == V_X_UPDATE) {
+        c1 = st->ac[k];
+        c2 = st->oc[k];
+    } else if constexpr (OP == V_R_UPDATE) {
+        c3 = st->omega[k];
+    } else if constexpr (OP == V_P_UPDATE) {
+        c1 = st->pcoef[k];
+        c2 = st->beta[k];
+    } else if constexpr (OP == V_GROUP5) {
+        c1 = st->ac[k];
+        c2 = st->oc[k];
+        c3 = st->beta[k] * st->omega[k];
+    }
+    for (int64_t i = start + tr; i < end; i += RPB) {
+        const size_t o = (size_t)i * NV + k;
+        if constexpr (OP == V_S_UPDATE) {
+            // axpby_z(a1, r, -alpha, v, s): s = 1*r + (-alpha)*v
+            s[o] = r[o] + c1 * v[o];
+        } else if constexpr (OP == V_XR_UPDATE) {
+            // axpbypcz(ac, p_, oc, s_, a1, x) then merged_bicgstab_group4
+            x[o] = c1 * p_[o] + c2 * s_[o] + x[o];
+            const double rv = s[o] - c3 * t[o];
+            r[o] = rv;
+            part[0] = part[0] + rv * r0[o];
+        } else if constexpr (OP == V_X_UPDATE) {
+            x[o] = c1 * p_[o] + c2 * s_[o] + x[o];
+        } else if constexpr (OP == V_R_UPDATE) {
+            const double rv = s[o] - c3 * t[o];
+            r[o] = rv;
+            part[0] = part[0] + rv * r0[o];
+        } else if constexpr (OP == V_P_UPDATE) {
+            // axpbypcz(a1, r, -(omega*beta), v, beta, p)
+            p[o] = r[o] + c1 * v[o] + c2 * p[o];
+        } else if constexpr (OP == V_GROUP5) {
+            x[o] += c1 * p[o] + c2 * s[o];
+            p[o] = r[o] + st->beta[k] * p[o] - c3 * v[o];
+        } else if constexpr (OP == V_DOT3) {
+            const double tv = t[o], sv = s[o];
+            part[0] = part[0] + tv * sv;
+            part[1] = part[1] + tv * tv;
+            part[2] = part[2] + sv * sv;
+        } else if constexpr (OP == V_DOT1) {
+            part[0] = part[0] + r[o] * r0[o];
+        }
+    }
+    if constexpr (ND > 0)
+        block_partials<NV, ND>(part, partials, lb);
+}
+
+template <int NV>
+int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
+    const int rpb = kBlock / NV;
+    const bool dots = (op == V_XR_UPDATE || op == V_R_UPDATE || op == V_DOT1 || op == V_DOT3);
+    int64_t need = (n + rpb - 1) / rpb;
+    int64_t g = std::min<int64_t>(need, dots ? kMaxPartials : 8192);
+    g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+    int64_t chunk = (n + g - 1) / g;
+    chunk = (chunk + rpb - 1) / rpb * rpb;
+#define XH_VEC_CASE(OPV)                                                                        \
+    case OPV:                                                                                   \
+        hipLaunchKernelGGL((k_vec<NV, OPV>), dim3(g), dim3(kBlock), 0, s, n, chunk,             \
+                           (double *)a.x, (double *)a.p, (double *)a.s, (double *)a.r,          \
+                           (const double *)a.r0, (const double *)a.v, (const double *)a.t,      \
+                           (const double *)a.p_, (const double *)a.s_, a.st, gate, a.partials); \
+        break;
+    switch (op) {
+        XH_VEC_CASE(V_S_UPDATE)
+        XH_VEC_CASE(V_XR_UPDATE)
+        XH_VEC_CASE(V_P_UPDATE)
+        XH_VEC_CASE(V_X_UPDATE)
+        XH_VEC_CASE(V_R_UPDATE)
+        XH_VEC_CASE(V_GROUP5)
+        XH_VEC_CASE(V_DOT3)
+        XH_VEC_CASE(V_DOT1)
+    default:
+        return -1;
+    }
+#undef XH_VEC_CASE
+    return dots ? (int)g : 0;
+}
+
+// ---- finalize: partial sums -> Krylov scalars and the convergence decision ----
+
+__device__ void conv_check(SolverState *st, const double *res, const double *res0, int k,
+                           int nv, int *s_notconv) {
+    // base_solver::convergence_check (solver.h:173-203), tolerances are float32
+    if (k < nv && st->iconv[k] != 0.0) {
+        const double a = sqrt(res[k]);
+        const double rr = a / sqrt(res0[k]);
+        if ((a > (double)st->abs_tol) && (rr > (double)st->rel_tol)) {
+            atomicOr(s_notconv, 1);
+        } else if (!st->conv[k]) {
+            st->abs_res[k] = a;
+            st->rel_res[k] = rr;
+            st->conv[k] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int op, SolverState *__restrict__ st,
+                                                  const double *__restrict__ partials,
+                                                  int nparts, int nv) {
+    if (st->done)
+        return;
+    const int nd = (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) ? 3 : 1;
+    const int C = nd * nv;
+    const int S = 256 / C; // >= 1 since nd*nv <= 3*64 < 256
+    __shared__ double red[256];
+    __shared__ double tot[3 * kMaxNV];
+    __shared__ int s_notconv;
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        s_notconv = 0;
+    if (tid < C * S) {
+        const int c = tid % C, sub = tid / C;
+        double acc = 0.0;
+        for (int g = sub; g < nparts; g += S)
+            acc = acc + partials[(size_t)g * C + c];
+        red[sub * C + c] = acc;
+    }
+    __syncthreads();
+    if (tid < C) {
+        double acc = red[tid];
+        for (int sub = 1; sub < S; ++sub)
+            acc = acc + red[sub * C + tid];
+        tot[tid] = acc;
+    }
+    __syncthreads();
+    const int k = tid;
+    const bool act = k < nv;
+    if (op == FIN_INIT) {
+        if (act) {
+            st->rho0[k] = tot[k];
+            if (st->hist && st->hist_len > 0)
+                st->hist[k] = sqrt(tot[k]);
+        }
+        __syncthreads();
+        if (st->conv_check)
+            conv_check(st, st->rho0, st->rho0, k, nv, &s_notconv);
+        __syncthreads();
+        if (!st->conv_check) {
+            if (act)
+                st->rho[k] = st->rho0[k];
+            return;
+        }
+        if (!s_notconv) {
+            if (tid == 0) {
+                st->conv_all = 1;
+                st->done = 1;
+            }
+            return;
+        }
+        if (act) {
+            st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
+            st->rho[k] = st->rho0[k];
+        }
+        return;
+    }
+    if (op == FIN_ALPHA) {
+        if (tid == 0)
+            st->iters += 1;
+        if (act) {
+            st->scal1[k] = tot[k];
+            const double al = st->rho[k] / tot[k]; // xdivy_z(rho, scal1, alpha)
+            st->alpha[k] = al;
+            st->nalpha[k] = al * -1.0;
+        }
+        return;
+    }
+    if (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) {
+        if (act) {
+            const double ph = tot[k], ps = tot[nv + k], th = tot[2 * nv + k];
+            st->phi[k] = ph;
+            st->psi[k] = ps;
+            st->theta[k] = th;
+            const double om = ph / ps;      // xdivy_z(phi, psi, omega)
+            st->omega[k] = om;
+            st->nomega[k] = om * -1.0;
+            const double wr = om * ph;      // ax_y(omega, phi, res)
+            st->res[k] = th + -1.0 * wr;    // axpby(a1, theta, a_1, res)
+            st->ac[k] = st->iconv[k] * st->alpha[k];
+            st->oc[k] = st->iconv[k] * om;
+            const int it = st->iters;
+            if (st->hist && it < st->hist_len)
+                st->hist[(size_t)it * nv + k] = sqrt(st->res[k]);
+        }
+        __syncthreads();
+        if (!st->conv_check)
+            return;
+        conv_check(st, st->res, st->rho0, k, nv, &s_notconv);
+        __syncthreads();
+        if (!s_notconv) {
+            if (tid == 0)
+                st->conv_all = 1;
+            return;
+        }
+        if (act) {
+            st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
+            if (op == FIN_OMEGA_MERGED) {
+                st->ac[k] = st->iconv[k] * st->alpha[k];
+                st->oc[k] = st->iconv[k] * st->omega[k];
+            }
+        }
+        return;
+    }
+    if (op == FIN_BETA) {
+        if (st->conv_all) {
+            if (tid == 0)
+                st->done = 1;
+            return;
+        }
+        if (act) {
+            const double rn = tot[k];
+            const double s2 = rn / st->rho[k];          // xdivy_z(scal1, rho, scal2)
+            const double s3 = st->alpha[k] / st->omega[k]; // xdivy_z(alpha, omega, scal3)
+            const double be = s2 * s3;                  // ax_y(scal2, scal3, beta)
+            st->beta[k] = be;
+            st->rho[k] = rn;                            // copy(scal1, rho)
+            const double ob = st->omega[k] * be;        // ax_y(omega, beta, scal1)
+            st->pcoef[k] = ob * -1.0;                   // scal(a_1, scal1)
+        }
+        if (tid == 0 && st->iters >= st->max_iters)
+            st->done = 1;
+        return;
+    }
+}
+
+// ---- smoother / coarse / conversion kernels ----
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_cheb_init_zero(int64_t count, int nv,
+                                                           const VT *__restrict__ b,
+                                                           const VT *__restrict__ isd, VT c,
+                                                           VT *__restrict__ r,
+                                                           VT *__restrict__ u, int last,
+                                                           const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    // get_residual with a zero-flagged x gives r = 1*b; r *= isd; t1 = c*r; t1 *= isd
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock) {
+        const VT di = isd[e / nv];
+        const VT rv = b[e] * di;
+        VT t1 = c * rv;
+        t1 = t1 * di;
+        if (!last)
+            r[e] = rv;
+        u[e] = t1;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_jacobi(int64_t count, int nv, VT *__restrict__ x,
+                                                   const VT *__restrict__ b,
+                                                   const VT *__restrict__ r,
+                                                   const VT *__restrict__ invd, VT relax,
+                                                   int x_zero, const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock) {
+        const VT d = invd[e / nv];
+        if (x_zero)
+            x[e] = relax * d * b[e];
+        else
+            x[e] += relax * d * (b[e] - r[e]);
+    }
+}
+
+// HSGS: one group of NV lanes sweeps one core block forward then backward; the block's
+// x lives in LDS, each lane only touches its own RHS column so no barrier is needed.
+template <typename VT, typename IT, int NV>
+__global__ __launch_bounds__(64) void k_hsgs_lds(const uint32_t *__restrict__ row,
+                                                 const IT *__restrict__ col,
+                                                 const VT *__restrict__ val,
+                                                 const uint32_t *__restrict__ pbase, int shift,
+                                                 const VT *__restrict__ invd, int64_t n,
+                                                 int64_t B, const VT *__restrict__ t,
+                                                 VT *__restrict__ x, int x_zero, VT relax,
+                                                 const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    extern __shared__ unsigned char smem[];
+    constexpr int G = 64 / NV;
+    const int g = threadIdx.x / NV, k = threadIdx.x % NV;
+    VT *xs = (VT *)smem + (size_t)g * B * NV;
+    const int64_t blk = (int64_t)blockIdx.x * G + g;
+    const int64_t lo = blk * B;
+    if (lo >= n)
+        return;
+    const int64_t hi = lo + B < n ? lo + B : n;
+    for (int64_t i = lo; i < hi; ++i)
+        xs[(i - lo) * NV + k] = x_zero ? VT(0) : x[(size_t)i * NV + k];
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (int64_t q = 0; q < hi - lo; ++q) {
+            const int64_t i = sweep == 0 ? lo + q : hi - 1 - q;
+            uint32_t base = 0;
+            if constexpr (sizeof(IT) == 2)
+                base = pbase[i >> shift];
+            VT sum = t[(size_t)i * NV + k];
+            for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
+                const int64_t c = (int64_t)(base + (uint32_t)col[j]);
+                if (c >= lo && c < hi)
+                    sum = sum - xs[(c - lo) * NV + k] * val[j];
+            }
+            const VT coeff = relax * invd[i];
+            xs[(i - lo) * NV + k] = xs[(i - lo) * NV + k] + coeff * sum;
+        }
+    }
+    for (int64_t i = lo; i < hi; ++i)
+        x[(size_t)i * NV + k] = xs[(i - lo) * NV + k];
+}
+
+// same sweep order on x in global memory, for blocks that do not fit in LDS
+template <typename VT, typename IT, int NV>
+__global__ __launch_bounds__(64) void k_hsgs_glb(const uint32_t *__restrict__ row,
+                                                 const IT *__restrict__ col,
+                                                 const VT *__restrict__ val,
+                                                 const uint32_t *__restrict__ pbase, int shift,
+                                                 const VT *__restrict__ invd, int64_t n,
+                                                 int64_t B, const VT *__restrict__ t, VT *x,
+                                                 int x_zero, VT relax,
+                                                 const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    constexpr int G = 64 / NV;
+    const int g = threadIdx.x / NV, k = threadIdx.x % NV;
+    const int64_t blk = (int64_t)blockIdx.x * G + g;
+    const int64_t lo = blk * B;
+    if (lo >= n)
+        return;
+    const int64_t hi = lo + B < n ? lo + B : n;
+    if (x_zero)
+        for (int64_t i = lo; i < hi; ++i)
+            x[(size_t)i * NV + k] = VT(0);
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (int64_t q = 0; q < hi - lo; ++q) {
+            const int64_t i = sweep == 0 ? lo + q : hi - 1 - q;
+            uint32_t base = 0;
+            if constexpr (sizeof(IT) == 2)
+                base = pbase[i >> shift];
+            VT sum = t[(size_t)i * NV + k];
+            for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
+                const int64_t c = (int64_t)(base + (uint32_t)col[j]);
+                if (c >= lo && c < hi)
+                    sum = sum - x[(size_t)c * NV + k] * val[j];
+            }
+            const VT coeff = relax * invd[i];
+            x[(size_t)i * NV + k] = x[(size_t)i * NV + k] + coeff * sum;
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_dense(int64_t n, int nv, const VT *__restrict__ D,
+                                                  const VT *__restrict__ x,
+                                                  VT *__restrict__ y,
+                                                  const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n * nv)
+        return;
+    const int64_t i = e / nv;
+    const int k = (int)(e % nv);
+    VT acc = VT(0);
+    for (int64_t j = 0; j < n; ++j)
+        acc = acc + D[i * n + j] * x[j * nv + k];
+    y[e] = acc;
+}
+
+template <typename T1, typename T2>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t count, const T1 *__restrict__ a,
+                                                    T2 *__restrict__ b,
+                                                    const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock)
+        b[e] = (T2)a[e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill0(int64_t count, T *__restrict__ b,
+                                                  const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock)
+        b[e] = T(0);
+}
+
+int grid_for(int64_t count) {
+    int64_t g = (count + kBlock - 1) / kBlock;
+    g = std::min<int64_t>(g, 16384);
+    return (int)std::max<int64_t>(1, g);
+}
+
+template <typename VT, typename IT, int NV>
+void hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int64_t B,
+            const int *done, hipStream_t s) {
+    constexpr int G = 64 / NV;
+    const int64_t nblk = (A.nrows + B - 1) / B;
+    const int grid = (int)((nblk + G - 1) / G);
+    const size_t lds = (size_t)64 * B * sizeof(VT);
+    if (lds <= 64 * 1024) {
+        hipLaunchKernelGGL((k_hsgs_lds<VT, IT, NV>), dim3(grid), dim3(64), lds, s, A.row,
+                           (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
+                           (const VT *)A.inv_diag, A.nrows, B, (const VT *)t, (VT *)x, x_zero,
+                           (VT)relax, done);
+    } else {
+        hipLaunchKernelGGL((k_hsgs_glb<VT, IT, NV>), dim3(grid), dim3(64), 0, s, A.row,
+                           (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
+                           (const VT *)A.inv_diag, A.nrows, B, (const VT *)t, (VT *)x, x_zero,
+                           (VT)relax, done);
+    }
+}
+
+template <typename VT, typename IT>
+void hsgs_nv(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax, int64_t B,
+             const int *done, hipStream_t s) {
+    switch (nv) {
+    case 1:
+        return hsgs_t<VT, IT, 1>(A, t, x, x_zero, relax, B, done, s);
+    case 2:
+        return hsgs_t<VT, IT, 2>(A, t, x, x_zero, relax, B, done, s);
+    case 4:
+        return hsgs_t<VT, IT, 4>(A, t, x, x_zero, relax, B, done, s);
+    case 8:
+        return hsgs_t<VT, IT, 8>(A, t, x, x_zero, relax, B, done, s);
+    case 16:
+        return hsgs_t<VT, IT, 16>(A, t, x, x_zero, relax, B, done, s);
+    case 32:
+        return hsgs_t<VT, IT, 32>(A, t, x, x_zero, relax, B, done, s);
+    case 64:
+        return hsgs_t<VT, IT, 64>(A, t, x, x_zero, relax, B, done, s);
+    default:
+        return;
+    }
+}
+
+} // namespace
+
+int vec_op(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s) {
+    const int gate = a.gate;
+    const VecArgs &b = a;
+    switch (nv) {
+    case 1:
+        return vec_nv<1>(op, n, b, gate, s);
+    case 2:
+        return vec_nv<2>(op, n, b, gate, s);
+    case 4:
+        return vec_nv<4>(op, n, b, gate, s);
+    case 8:
+        return vec_nv<8>(op, n, b, gate, s);
+    case 16:
+        return vec_nv<16>(op, n, b, gate, s);
+    case 32:
+        return vec_nv<32>(op, n, b, gate, s);
+    case 64:
+        return vec_nv<64>(op, n, b, gate, s);
+    default:
+        return -1;
+    }
+}
+
+void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int nv,
+              hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, (int)op, st, partials, nparts, nv);
+}
+
+void cheb_init_zero(int64_t n, int nv, int prec, const void *b, const void *isd, double c,
+                    void *r, void *u_or_x, int last, const int *done, hipStream_t s) {
+    const int64_t count = n * nv;
+    if (prec == F64)
+        hipLaunchKernelGGL(k_cheb_init_zero<double>, dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, nv, (const double *)b, (const double *)isd, c, (double *)r,
+                           (double *)u_or_x, last, done);
+    else
+        hipLaunchKernelGGL(k_cheb_init_zero<float>, dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, nv, (const float *)b, (const float *)isd, (float)c,
+                           (float *)r, (float *)u_or_x, last, done);
+}
+
+void jacobi_update(int64_t n, int nv, int prec, void *x, const void *b, const void *r,
+                   const void *inv_diag, double relax, int x_zero, const int *done,
+                   hipStream_t s) {
+    const int64_t count = n * nv;
+    if (prec == F64)
+        hipLaunchKernelGGL(k_jacobi<double>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           nv, (double *)x, (const double *)b, (const double *)r,
+                           (const double *)inv_diag, relax, x_zero, done);
+    else
+        hipLaunchKernelGGL(k_jacobi<float>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           nv, (float *)x, (const float *)b, (const float *)r,
+                           (const float *)inv_diag, (float)relax, x_zero, done);
+}
+
+int64_t hsgs_block_rows(int64_t n, int nblocks) {
+    // misc::split_range<float64_t> (src/misc/misc.h:46-55)
+    int64_t cs = n / nblocks;
+    cs += 8 - (cs % 8);
+    return cs;
+}
+
+void hsgs_sweeps(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax,
+                 int64_t B, const int *done, hipStream_t s) {
+    if (A.prec == F64) {
+        if (A.idx == IDX_U16)
+            hsgs_nv<double, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s);
+        else
+            hsgs_nv<double, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
+    } else {
+        if (A.idx == IDX_U16)
+            hsgs_nv<float, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s);
+        else
+            hsgs_nv<float, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
+    }
+}
+
+void dense_matvec(int64_t n, int nv, int prec, const void *D, const void *x, void *y,
+                  const int *done, hipStream_t s) {
+    const int64_t count = n * nv;
+    const int grid = (int)((count + kBlock - 1) / kBlock);
+    if (prec == F64)
+        hipLaunchKernelGGL(k_dense<double>, dim3(grid), dim3(kBlock), 0, s, n, nv,
+                           (const double *)D, (const double *)x, (double *)y, done);
+    else
+        hipLaunchKernelGGL(k_dense<float>, dim3(grid), dim3(kBlock), 0, s, n, nv,
+                           (const float *)D, (const float *)x, (float *)y, done);
+}
+
+void convert(int64_t count, int from_prec, const void *src, void *dst, const int *done,
+             hipStream_t s) {
+    if (from_prec == F64)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, (const double *)src, (float *)dst, done);
+    else
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, (const float *)src, (double *)dst, done);
+}
+
+void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *done,
+              hipStream_t s) {
+    if (prec == F64)
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(count)), dim3(kBlock), 0,
+                           s, count, (const double *)src, (double *)dst, done);
+    else
+        hipLaunchKernelGGL((k_convert<float, float>), dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, (const float *)src, (float *)dst, done);
+}
+
+void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s) {
+    if (prec == F64)
+        hipLaunchKernelGGL(k_fill0<double>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           (double *)dst, done);
+    else
+        hipLaunchKernelGGL(k_fill0<float>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           (float *)dst, done);
+}
+
+} // namespace xh

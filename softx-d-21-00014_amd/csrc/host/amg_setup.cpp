@@ -1,0 +1,488 @@
+// Classical (Ruge-Stueben) AMG setup on the host.
+//
+// The reference takes its hierarchy from hypre 2.20.0 BoomerAMG (not in the reference
+// tree and not available offline here).  This file restates the published classical
+// algorithm with the reference's default keys (src/param/xamg_params_details.h:179-209):
+//   strength      -a_ij >= theta * max_k(-a_ik) (sign relative to the diagonal),
+//                 rows whose |row sum| > max_row_sum * |a_ii| have no strong links
+//   coarsening    Ruge-Stueben first pass (measure = number of points that strongly
+//                 depend on a point), optional second pass
+//   interpolation modified classical interpolation (hypre interp type 0)
+//   truncation    drop |w| < trunc_factor * max|w|, keep at most Pmax entries,
+//                 rescale each row to its original sum
+//   Galerkin      A_c = R A P with R = P^T (hypre_base.inl:400-401)
+// Rows of every output matrix are stored diagonal first, then ascending column.
+#include "amg_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace xamg {
+namespace amg {
+
+void sort_rows_diag_first(CsrHost &m) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < m.nrows; ++i) {
+        uint32_t b = m.row[i], e = m.row[i + 1];
+        std::vector<std::pair<uint32_t, double>> r;
+        r.reserve(e - b);
+        for (uint32_t j = b; j < e; ++j)
+            r.emplace_back(m.col[j], m.val[j]);
+        std::sort(r.begin(), r.end(), [i](const std::pair<uint32_t, double> &x,
+                                          const std::pair<uint32_t, double> &y) {
+            bool dx = x.first == (uint32_t)i, dy = y.first == (uint32_t)i;
+            if (dx != dy)
+                return dx;
+            return x.first < y.first;
+        });
+        for (uint32_t j = b; j < e; ++j) {
+            m.col[j] = r[j - b].first;
+            m.val[j] = r[j - b].second;
+        }
+    }
+}
+
+CsrHost transpose(const CsrHost &m) {
+    // counting-sort transpose: entries of a row of the result appear in ascending
+    // source-row order (src/primitives/matrix/operations/transpose.inl:31-94)
+    CsrHost t;
+    t.nrows = m.ncols;
+    t.ncols = m.nrows;
+    t.row.assign(t.nrows + 1, 0);
+    for (uint32_t c : m.col)
+        t.row[c + 1]++;
+    for (int64_t i = 0; i < t.nrows; ++i)
+        t.row[i + 1] += t.row[i];
+    t.col.resize(m.col.size());
+    t.val.resize(m.col.size());
+    std::vector<uint32_t> pos(t.row.begin(), t.row.end() - 1);
+    for (int64_t i = 0; i < m.nrows; ++i)
+        for (uint32_t j = m.row[i]; j < m.row[i + 1]; ++j) {
+            uint32_t c = m.col[j];
+            t.col[pos[c]] = (uint32_t)i;
+            t.val[pos[c]] = m.val[j];
+            pos[c]++;
+        }
+    return t;
+}
+
+CsrHost spgemm(const CsrHost &a, const CsrHost &b) {
+    CsrHost c;
+    c.nrows = a.nrows;
+    c.ncols = b.ncols;
+    c.row.assign(a.nrows + 1, 0);
+    const int64_t n = a.nrows;
+    std::vector<std::vector<uint32_t>> rc(n);
+    std::vector<std::vector<double>> rv(n);
+#pragma omp parallel
+    {
+        std::vector<int64_t> mark(b.ncols, -1);
+        std::vector<double> acc(b.ncols, 0.0);
+        std::vector<uint32_t> cols;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < n; ++i) {
+            cols.clear();
+            for (uint32_t ja = a.row[i]; ja < a.row[i + 1]; ++ja) {
+                uint32_t k = a.col[ja];
+                double av = a.val[ja];
+                for (uint32_t jb = b.row[k]; jb < b.row[k + 1]; ++jb) {
+                    uint32_t cc = b.col[jb];
+                    if (mark[cc] != i) {
+                        mark[cc] = i;
+                        acc[cc] = av * b.val[jb];
+                        cols.push_back(cc);
+                    } else {
+                        acc[cc] += av * b.val[jb];
+                    }
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            rc[i] = cols;
+            rv[i].resize(cols.size());
+            for (size_t q = 0; q < cols.size(); ++q)
+                rv[i][q] = acc[cols[q]];
+        }
+    }
+    for (int64_t i = 0; i < n; ++i)
+        c.row[i + 1] = c.row[i] + (uint32_t)rc[i].size();
+    c.col.resize(c.row[n]);
+    c.val.resize(c.row[n]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        std::copy(rc[i].begin(), rc[i].end(), c.col.begin() + c.row[i]);
+        std::copy(rv[i].begin(), rv[i].end(), c.val.begin() + c.row[i]);
+    }
+    return c;
+}
+
+namespace {
+
+double diag_of(const CsrHost &A, int64_t i) {
+    for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+        if (A.col[j] == (uint32_t)i)
+            return A.val[j];
+    return 0.0;
+}
+
+// strength pattern S: for every row, a flag per nonzero (1 = strong dependency)
+std::vector<uint8_t> strength(const CsrHost &A, float theta, float max_row_sum) {
+    std::vector<uint8_t> s(A.col.size(), 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < A.nrows; ++i) {
+        double d = diag_of(A, i);
+        double sgn = d < 0 ? -1.0 : 1.0;
+        double mx = 0.0, rs = 0.0;
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+            rs += A.val[j];
+            if (A.col[j] == (uint32_t)i)
+                continue;
+            mx = std::max(mx, -sgn * A.val[j]);
+        }
+        if (mx <= 0.0)
+            continue;
+        if (max_row_sum < 1.0f && std::fabs(rs) > (double)max_row_sum * std::fabs(d))
+            continue;
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+            if (A.col[j] == (uint32_t)i)
+                continue;
+            if (-sgn * A.val[j] >= (double)theta * mx)
+                s[j] = 1;
+        }
+    }
+    return s;
+}
+
+enum : int8_t { UNDECIDED = 0, CPT = 1, FPT = -1 };
+
+// Ruge-Stueben first pass with bucket lists keyed by the measure
+std::vector<int8_t> rs_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, int second_pass) {
+    const int64_t n = A.nrows;
+    // S^T adjacency: who strongly depends on i
+    std::vector<uint32_t> tr(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+            if (S[j])
+                tr[A.col[j] + 1]++;
+    for (int64_t i = 0; i < n; ++i)
+        tr[i + 1] += tr[i];
+    std::vector<uint32_t> tc(tr[n]);
+    {
+        std::vector<uint32_t> pos(tr.begin(), tr.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+                if (S[j])
+                    tc[pos[A.col[j]]++] = (uint32_t)i;
+    }
+    std::vector<int8_t> cf(n, UNDECIDED);
+    std::vector<int64_t> lam(n);
+    int64_t maxlam = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        lam[i] = tr[i + 1] - tr[i];
+        bool has_dep = false;
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+            has_dep |= S[j] != 0;
+        if (!has_dep && lam[i] == 0)
+            cf[i] = FPT; // isolated point: nothing to interpolate from
+        maxlam = std::max(maxlam, lam[i]);
+    }
+    const int64_t nb = 2 * maxlam + n + 2; // measures can grow while F points are made
+    std::vector<int64_t> head(nb, -1), nxt(n, -1), prv(n, -1);
+    auto insert = [&](int64_t i) {
+        int64_t b = lam[i];
+        nxt[i] = head[b];
+        prv[i] = -1;
+        if (head[b] >= 0)
+            prv[head[b]] = i;
+        head[b] = i;
+    };
+    auto remove = [&](int64_t i) {
+        int64_t b = lam[i];
+        if (prv[i] >= 0)
+            nxt[prv[i]] = nxt[i];
+        else
+            head[b] = nxt[i];
+        if (nxt[i] >= 0)
+            prv[nxt[i]] = prv[i];
+        nxt[i] = prv[i] = -1;
+    };
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == UNDECIDED)
+            insert(i);
+    int64_t top = nb - 1;
+    while (true) {
+        while (top >= 0 && head[top] < 0)
+            --top;
+        if (top < 0)
+            break;
+        int64_t i = head[top];
+        if (lam[i] == 0) {
+            // nobody depends on the remaining points: they become F points
+            for (int64_t k = 0; k < n; ++k)
+                if (cf[k] == UNDECIDED)
+                    cf[k] = FPT;
+            break;
+        }
+        remove(i);
+        cf[i] = CPT;
+        for (uint32_t q = tr[i]; q < tr[i + 1]; ++q) {
+            int64_t j = tc[q];
+            if (cf[j] != UNDECIDED)
+                continue;
+            remove(j);
+            cf[j] = FPT;
+            for (uint32_t jj = A.row[j]; jj < A.row[j + 1]; ++jj) {
+                if (!S[jj])
+                    continue;
+                int64_t k = A.col[jj];
+                if (cf[k] != UNDECIDED)
+                    continue;
+                remove(k);
+                lam[k]++;
+                if (lam[k] >= nb)
+                    lam[k] = nb - 1;
+                insert(k);
+                top = std::max(top, lam[k]);
+            }
+        }
+        for (uint32_t jj = A.row[i]; jj < A.row[i + 1]; ++jj) {
+            if (!S[jj])
+                continue;
+            int64_t k = A.col[jj];
+            if (cf[k] != UNDECIDED)
+                continue;
+            remove(k);
+            if (lam[k] > 0)
+                lam[k]--;
+            insert(k);
+        }
+    }
+    if (second_pass) {
+        // F points i, j with a strong F-F link must share a strong C point
+        std::vector<int64_t> mark(n, -1);
+        for (int64_t i = 0; i < n; ++i) {
+            if (cf[i] != FPT)
+                continue;
+            for (uint32_t jj = A.row[i]; jj < A.row[i + 1]; ++jj)
+                if (S[jj] && cf[A.col[jj]] == CPT)
+                    mark[A.col[jj]] = i;
+            for (uint32_t jj = A.row[i]; jj < A.row[i + 1]; ++jj) {
+                if (!S[jj])
+                    continue;
+                int64_t k = A.col[jj];
+                if (cf[k] != FPT)
+                    continue;
+                bool common = false;
+                for (uint32_t kk = A.row[k]; kk < A.row[k + 1]; ++kk)
+                    if (S[kk] && cf[A.col[kk]] == CPT && mark[A.col[kk]] == i) {
+                        common = true;
+                        break;
+                    }
+                if (!common) {
+                    cf[k] = CPT;
+                    mark[k] = i;
+                }
+            }
+        }
+    }
+    return cf;
+}
+
+// modified classical interpolation + truncation
+CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
+                      const std::vector<int8_t> &cf, const SetupParams &p, int64_t &nc) {
+    const int64_t n = A.nrows;
+    std::vector<int64_t> cidx(n, -1);
+    nc = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT)
+            cidx[i] = nc++;
+    std::vector<std::vector<uint32_t>> rc(n);
+    std::vector<std::vector<double>> rv(n);
+#pragma omp parallel
+    {
+        std::vector<int64_t> mark(n, -1);
+        std::vector<double> w(n, 0.0);
+#pragma omp for schedule(dynamic, 512)
+        for (int64_t i = 0; i < n; ++i) {
+            if (cf[i] == CPT) {
+                rc[i] = {(uint32_t)cidx[i]};
+                rv[i] = {1.0};
+                continue;
+            }
+            std::vector<uint32_t> Ci;
+            double diag = 0.0;
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+                uint32_t c = A.col[j];
+                if (c == (uint32_t)i)
+                    diag += A.val[j];
+                else if (S[j] && cf[c] == CPT) {
+                    Ci.push_back(c);
+                    mark[c] = i;
+                    w[c] = A.val[j];
+                }
+            }
+            if (Ci.empty())
+                continue; // F point with no strong C neighbour: empty row of P
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+                uint32_t k = A.col[j];
+                if (k == (uint32_t)i || (S[j] && cf[k] == CPT))
+                    continue;
+                if (!S[j]) {
+                    diag += A.val[j]; // weak connection: lumped into the diagonal
+                    continue;
+                }
+                // strong F neighbour k: distribute a_ik over C_i with weights abar_km
+                double akk = diag_of(A, k);
+                double den = 0.0;
+                for (uint32_t q = A.row[k]; q < A.row[k + 1]; ++q) {
+                    uint32_t m = A.col[q];
+                    if (m < (uint32_t)n && mark[m] == i && A.val[q] * akk < 0)
+                        den += A.val[q];
+                }
+                if (den == 0.0) {
+                    diag += A.val[j];
+                    continue;
+                }
+                for (uint32_t q = A.row[k]; q < A.row[k + 1]; ++q) {
+                    uint32_t m = A.col[q];
+                    if (mark[m] == i && A.val[q] * akk < 0)
+                        w[m] += A.val[j] * A.val[q] / den;
+                }
+            }
+            std::vector<std::pair<uint32_t, double>> row;
+            for (uint32_t c : Ci)
+                row.emplace_back(c, diag != 0.0 ? -w[c] / diag : 0.0);
+            // truncation (trunc factor, then Pmax), each followed by a rescale to the
+            // original row sum
+            double sum0 = 0.0, mx = 0.0;
+            for (auto &e : row) {
+                sum0 += e.second;
+                mx = std::max(mx, std::fabs(e.second));
+            }
+            if (p.trunc_factor > 0.0f) {
+                std::vector<std::pair<uint32_t, double>> kept;
+                for (auto &e : row)
+                    if (std::fabs(e.second) >= (double)p.trunc_factor * mx)
+                        kept.push_back(e);
+                double s1 = 0.0;
+                for (auto &e : kept)
+                    s1 += e.second;
+                if (s1 != 0.0 && kept.size() < row.size())
+                    for (auto &e : kept)
+                        e.second *= sum0 / s1;
+                row.swap(kept);
+            }
+            if (p.pmax > 0 && (int)row.size() > p.pmax) {
+                double s0 = 0.0;
+                for (auto &e : row)
+                    s0 += e.second;
+                std::stable_sort(row.begin(), row.end(),
+                                 [](const std::pair<uint32_t, double> &a,
+                                    const std::pair<uint32_t, double> &b) {
+                                     return std::fabs(a.second) > std::fabs(b.second);
+                                 });
+                row.resize(p.pmax);
+                double s1 = 0.0;
+                for (auto &e : row)
+                    s1 += e.second;
+                if (s1 != 0.0)
+                    for (auto &e : row)
+                        e.second *= s0 / s1;
+            }
+            std::sort(row.begin(), row.end());
+            for (auto &e : row) {
+                rc[i].push_back((uint32_t)cidx[e.first]);
+                rv[i].push_back(e.second);
+            }
+        }
+    }
+    CsrHost P;
+    P.nrows = n;
+    P.ncols = nc;
+    P.row.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        P.row[i + 1] = P.row[i] + (uint32_t)rc[i].size();
+    P.col.resize(P.row[n]);
+    P.val.resize(P.row[n]);
+    for (int64_t i = 0; i < n; ++i) {
+        std::copy(rc[i].begin(), rc[i].end(), P.col.begin() + P.row[i]);
+        std::copy(rv[i].begin(), rv[i].end(), P.val.begin() + P.row[i]);
+    }
+    return P;
+}
+
+uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+} // namespace
+
+void estimate_eigs(const CsrHost &A, int iters, uint64_t seed, double &mn, double &mx) {
+    const int64_t n = A.nrows;
+    std::vector<double> isd(n), x(n), y(n);
+    for (int64_t i = 0; i < n; ++i) {
+        double d = diag_of(A, i);
+        isd[i] = d != 0.0 ? std::sqrt(1.0 / d) : 0.0;
+        uint64_t z = splitmix64(seed * 0x100000001B3ull + (uint64_t)i);
+        x[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    }
+    double lam = 0.0;
+    for (int it = 0; it < std::max(1, iters); ++it) {
+        double xy = 0.0, xx = 0.0, yy = 0.0;
+#pragma omp parallel for reduction(+ : xy, xx, yy) schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+                s += A.val[j] * isd[A.col[j]] * x[A.col[j]];
+            y[i] = isd[i] * s;
+            xy += x[i] * y[i];
+            xx += x[i] * x[i];
+            yy += y[i] * y[i];
+        }
+        lam = xx > 0 ? xy / xx : 0.0;
+        double nrm = std::sqrt(yy);
+        if (nrm == 0.0)
+            break;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            x[i] = y[i] / nrm;
+    }
+    mx = lam;
+    mn = 0.0;
+}
+
+void build(const CsrHost &A0, const SetupParams &p, Hierarchy &H) {
+    H = Hierarchy();
+    CsrHost A = A0;
+    sort_rows_diag_first(A);
+    H.A.push_back(A);
+    while ((int)H.A.size() < p.max_levels && H.A.back().nrows > p.coarse_size) {
+        const CsrHost &Af = H.A.back();
+        std::vector<uint8_t> S = strength(Af, p.strength, p.max_row_sum);
+        std::vector<int8_t> cf = rs_coarsen(Af, S, p.second_pass);
+        int64_t nc = 0;
+        CsrHost P = interpolation(Af, S, cf, p, nc);
+        if (nc == 0 || nc >= Af.nrows)
+            break;
+        CsrHost R = transpose(P);
+        CsrHost AP = spgemm(Af, P);
+        CsrHost Ac = spgemm(R, AP);
+        sort_rows_diag_first(Ac);
+        H.P.push_back(std::move(P));
+        H.R.push_back(std::move(R));
+        H.A.push_back(std::move(Ac));
+    }
+    H.min_eig.assign(H.A.size(), 0.0);
+    H.max_eig.assign(H.A.size(), 0.0);
+    if (p.eig_iters > 0)
+        for (size_t l = 0; l < H.A.size(); ++l)
+            estimate_eigs(H.A[l], p.eig_iters, p.eig_seed, H.min_eig[l], H.max_eig[l]);
+}
+
+} // namespace amg
+} // namespace xamg

@@ -1,0 +1,48 @@
+// Host-side classical AMG setup: builds the multigrid hierarchy that the reference
+// obtains from hypre BoomerAMG (src/hypre/detail/hypre_hierarchy.inl:31-52,
+// src/hypre/detail/hypre_base.inl:337-403).  Runs once per matrix, before the solve
+// phase; its output (A_l, P_l, R_l = P_l^T, eigenvalue estimates) is the input of the
+// GPU solve.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace xamg {
+namespace amg {
+
+struct CsrHost {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<uint32_t> row, col;
+    std::vector<double> val;
+    int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+struct SetupParams {
+    float strength = 0.5f;    // mg_strength_threshold
+    float max_row_sum = 0.9f; // mg_max_row_sum
+    int coarse_size = 100;    // mg_coarse_matrix_size
+    int max_levels = 25;      // mg_max_levels
+    float trunc_factor = 0.25f; // mg_trunc_factor
+    int pmax = 4;             // mg_Pmax_elements
+    int second_pass = 0;      // RS second pass (F-F strong pairs share a C point)
+    int eig_iters = 10;       // power iterations for the Chebyshev bounds
+    uint64_t eig_seed = 12345;
+};
+
+struct Hierarchy {
+    std::vector<CsrHost> A, P, R;
+    std::vector<double> min_eig, max_eig;
+    int nlevels() const { return (int)A.size(); }
+};
+
+// rows reordered in place: diagonal first, then ascending column
+void sort_rows_diag_first(CsrHost &m);
+CsrHost transpose(const CsrHost &m);
+CsrHost spgemm(const CsrHost &a, const CsrHost &b);
+void build(const CsrHost &A0, const SetupParams &p, Hierarchy &H);
+// lambda_max of D^{-1/2} A D^{-1/2} by power iteration; lambda_min reported as 0
+void estimate_eigs(const CsrHost &A, int iters, uint64_t seed, double &mn, double &mx);
+
+} // namespace amg
+} // namespace xamg

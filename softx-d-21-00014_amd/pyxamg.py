@@ -1,0 +1,410 @@
+"""ctypes bindings of libxamg_hip.so (include/xamg_hip.h) and libxamg.so
+(include/xamg_amg.h, include/xamg_c_api.h) for the test suite and the benchmark.
+
+Loads the in-tree shared libraries under lib/ and fails loudly when they are missing:
+there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+# method ids, include/xamg_hip.h
+IDENTITY, JACOBI, HSGS, CHEBYSHEV, DIRECT, MULTIGRID = 0, 1, 2, 3, 4, 5
+BICGSTAB, PBICGSTAB, MERGED_BICGSTAB, MERGED_PBICGSTAB = 6, 7, 8, 9
+NONE = -1
+
+METHODS = {
+    "Identity": IDENTITY, "Jacobi": JACOBI, "HSGS": HSGS, "Chebyshev": CHEBYSHEV,
+    "Direct": DIRECT, "MultiGrid": MULTIGRID, "BiCGStab": BICGSTAB, "PBiCGStab": PBICGSTAB,
+    "MergedBiCGStab": MERGED_BICGSTAB, "MergedPBiCGStab": MERGED_PBICGSTAB,
+}
+
+
+class Role(C.Structure):
+    _fields_ = [("method", C.c_int32), ("max_iters", C.c_int32), ("abs_tol", C.c_float),
+                ("rel_tol", C.c_float), ("conv_check", C.c_int32), ("relax_factor", C.c_float),
+                ("poly_order", C.c_int32), ("spectrum_fraction", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [("solver", Role), ("precond", Role), ("pre", Role), ("post", Role),
+                ("coarse", Role), ("mg_cycle", C.c_int32), ("mg_prec", C.c_int32),
+                ("hsgs_ncores", C.c_int32), ("hsgs_block_rows", C.c_int32),
+                ("coarse_fp64", C.c_int32), ("index_mode", C.c_int32), ("graph", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
+class AmgParams(C.Structure):
+    _fields_ = [("strength", C.c_float), ("max_row_sum", C.c_float), ("coarse_size", C.c_int32),
+                ("max_levels", C.c_int32), ("trunc_factor", C.c_float), ("pmax", C.c_int32),
+                ("second_pass", C.c_int32), ("eig_iters", C.c_int32), ("eig_seed", C.c_uint64)]
+
+
+def role(method, max_iters=1, abs_tol=0.0, rel_tol=0.0, conv_check=0, relax_factor=0.99,
+         poly_order=2, spectrum_fraction=0.3):
+    """One parameter role with the reference defaults of a non-solver list."""
+    m = METHODS[method] if isinstance(method, str) else method
+    return Role(m, max_iters, abs_tol, rel_tol, conv_check, relax_factor, poly_order,
+                spectrum_fraction)
+
+
+_hip = None
+_host = None
+
+
+class XamgError(RuntimeError):
+    pass
+
+
+def _load():
+    global _hip, _host
+    if _hip is not None:
+        return
+    hip_path = os.path.join(LIBDIR, "libxamg_hip.so")
+    host_path = os.path.join(LIBDIR, "libxamg.so")
+    for p in (hip_path, host_path):
+        if not os.path.exists(p):
+            raise XamgError(f"{p} is not built: run __graft_entry__.build()")
+    _hip = C.CDLL(hip_path, mode=C.RTLD_GLOBAL)
+    _host = C.CDLL(host_path, mode=C.RTLD_GLOBAL)
+    vp, i32, i64, f64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64
+    P = C.POINTER
+    _hip.xh_last_error.restype = C.c_char_p
+    sig = {
+        "xh_init": [C.c_int], "xh_finalize": [], "xh_malloc": [P(vp), C.c_size_t],
+        "xh_free": [vp], "xh_memcpy_h2d": [vp, vp, C.c_size_t],
+        "xh_memcpy_d2h": [vp, vp, C.c_size_t], "xh_synchronize": [],
+        "xh_device_info": [C.c_char_p, C.c_int, P(C.c_int), P(f64)],
+        "xh_matrix_create": [i64, i64, i64, vp, vp, vp, C.c_int, C.c_int, P(vp)],
+        "xh_matrix_info": [vp, P(i64), P(i64), P(i64), P(C.c_int), P(C.c_int), P(i64),
+                           P(C.c_size_t)],
+        "xh_matrix_destroy": [vp],
+        "xh_spmv": [vp, C.c_int, vp, vp, C.c_int],
+        "xh_spmv_timed": [vp, C.c_int, vp, vp, C.c_int, C.c_int, P(f64)],
+        "xh_dense_matvec": [i64, C.c_int, C.c_int, vp, vp, vp],
+        "xh_hier_create": [C.c_int, P(vp)],
+        "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
+                              vp, vp, f64, f64],
+        "xh_hier_destroy": [vp],
+        "xh_solver_create": [vp, P(Params), C.c_int, P(vp)],
+        "xh_solver_solve": [vp, vp, vp], "xh_solver_solve_device": [vp, vp, vp],
+        "xh_solver_stats": [vp, P(i32), vp, vp, vp],
+        "xh_solver_history": [vp, vp, i32, P(i32)],
+        "xh_solver_precond_apply": [vp, vp, vp],
+        "xh_solver_renew": [vp, P(Role)],
+        "xh_solver_info": [vp, P(C.c_int), P(C.c_size_t)],
+        "xh_solver_destroy": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(_hip, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    hsig = {
+        "xamg_amg_setup": [i64, vp, vp, vp, P(AmgParams), P(vp)],
+        "xamg_amg_nlevels": [vp],
+        "xamg_amg_level": [vp, C.c_int, C.c_int, P(i64), P(i64), P(i64), P(vp), P(vp), P(vp)],
+        "xamg_amg_eigs": [vp, C.c_int, P(f64), P(f64)],
+        "xamg_amg_estimate_eigs": [vp, C.c_int, u64],
+        "xamg_amg_to_device": [vp, P(vp)],
+        "xamg_amg_free": [vp],
+        "xamg_cube": [i64, i64, i64, vp, vp, vp],
+        "xamg_rhs_uniform": [u64, i64, vp],
+    }
+    for name, args in hsig.items():
+        f = getattr(_host, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _host.xamg_cube_nnz.argtypes = [i64, i64, i64]
+    _host.xamg_cube_nnz.restype = i64
+    _host.xamg_amg_default_params.argtypes = [P(AmgParams)]
+    _host.xamg_amg_default_params.restype = None
+    _host.XAMG_get_last_error.restype = C.c_char_p
+
+
+def hip():
+    _load()
+    return _hip
+
+
+def host():
+    _load()
+    return _host
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = _hip.xh_last_error().decode() if _hip is not None else ""
+        hmsg = _host.XAMG_get_last_error().decode() if _host is not None else ""
+        raise XamgError(f"{what} failed ({rc}): {msg or hmsg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------- host-side helpers ----------------
+
+def cube(nx, ny=None, nz=None):
+    """7-point cube Laplacian of the reference generator as (row, col, val) numpy arrays."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    h = host()
+    nnz = h.xamg_cube_nnz(nx, ny, nz)
+    n = nx * ny * nz
+    row = np.zeros(n + 1, np.uint32)
+    col = np.zeros(nnz, np.uint32)
+    val = np.zeros(nnz, np.float64)
+    check(h.xamg_cube(nx, ny, nz, _ptr(row), _ptr(col), _ptr(val)), "xamg_cube")
+    return row, col, val
+
+
+def rhs_uniform(seed, n, nv):
+    b = np.zeros(n * nv, np.float64)
+    check(host().xamg_rhs_uniform(seed, n * nv, _ptr(b)), "xamg_rhs_uniform")
+    return b
+
+
+def amg_default_params():
+    p = AmgParams()
+    host().xamg_amg_default_params(C.byref(p))
+    return p
+
+
+class Amg:
+    """Host-side classical AMG hierarchy (libxamg.so)."""
+
+    def __init__(self, row, col, val, params=None, **kw):
+        p = params or amg_default_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._keep = (row, col, val)
+        self.h = C.c_void_p()
+        n = len(row) - 1
+        check(host().xamg_amg_setup(n, _ptr(row), _ptr(col), _ptr(val), C.byref(p),
+                                    C.byref(self.h)), "xamg_amg_setup")
+
+    @property
+    def nlevels(self):
+        return host().xamg_amg_nlevels(self.h)
+
+    def level(self, l, which):
+        """which: 0 A, 1 P, 2 R -> (nrows, ncols, row, col, val) copies"""
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        r, c, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(host().xamg_amg_level(self.h, l, which, C.byref(nr), C.byref(nc), C.byref(nnz),
+                                    C.byref(r), C.byref(c), C.byref(v)), "xamg_amg_level")
+        row = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_uint32)), (nr.value + 1,)).copy()
+        col = (np.ctypeslib.as_array(C.cast(c, C.POINTER(C.c_uint32)), (nnz.value,)).copy()
+               if nnz.value else np.zeros(0, np.uint32))
+        val = (np.ctypeslib.as_array(C.cast(v, C.POINTER(C.c_double)), (nnz.value,)).copy()
+               if nnz.value else np.zeros(0, np.float64))
+        return nr.value, nc.value, row, col, val
+
+    def eigs(self, l):
+        a, b = C.c_double(), C.c_double()
+        check(host().xamg_amg_eigs(self.h, l, C.byref(a), C.byref(b)), "xamg_amg_eigs")
+        return a.value, b.value
+
+    def estimate_eigs(self, iters=10, seed=12345):
+        check(host().xamg_amg_estimate_eigs(self.h, iters, seed), "xamg_amg_estimate_eigs")
+
+    def export(self):
+        """list of dicts with A, P, R (scipy-free numpy CSR triplets) and eigenvalues"""
+        out = []
+        for l in range(self.nlevels):
+            d = {"A": self.level(l, 0), "eig": self.eigs(l)}
+            if l + 1 < self.nlevels:
+                d["P"] = self.level(l, 1)
+                d["R"] = self.level(l, 2)
+            out.append(d)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _host is not None:
+            _host.xamg_amg_free(self.h)
+            self.h = None
+
+
+class Hier:
+    """Device-side hierarchy object (libxamg_hip.so) built from exported levels."""
+
+    def __init__(self, levels):
+        self.h = C.c_void_p()
+        check(hip().xh_hier_create(len(levels), C.byref(self.h)), "xh_hier_create")
+        self._keep = levels
+        for l, d in enumerate(levels):
+            n, _, ra, ca, va = d["A"]
+            if "P" in d:
+                _, nc, rp, cp, vp_ = d["P"]
+                _, _, rr, cr, vr = d["R"]
+                args = (nc, len(cp), _ptr(rp), _ptr(cp), _ptr(vp_), len(cr), _ptr(rr), _ptr(cr),
+                        _ptr(vr))
+            else:
+                args = (0, 0, None, None, None, 0, None, None, None)
+            mn, mx = d["eig"]
+            check(hip().xh_hier_set_level(self.h, l, n, len(ca), _ptr(ra), _ptr(ca), _ptr(va),
+                                          *args, mn, mx), "xh_hier_set_level")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _hip is not None:
+            _hip.xh_hier_destroy(self.h)
+            self.h = None
+
+
+class Solver:
+    def __init__(self, hier, params, nv):
+        self.hier = hier
+        self.nv = nv
+        self.n = hier._keep[0]["A"][0]
+        self.s = C.c_void_p()
+        check(hip().xh_solver_create(hier.h, C.byref(params), nv, C.byref(self.s)),
+              "xh_solver_create")
+
+    def solve(self, b, x0=None):
+        b = np.ascontiguousarray(b, np.float64).reshape(-1)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        check(hip().xh_solver_solve(self.s, _ptr(b), _ptr(x)), "xh_solver_solve")
+        return x
+
+    def solve_device(self, db, dx):
+        check(hip().xh_solver_solve_device(self.s, db.ptr, dx.ptr), "xh_solver_solve_device")
+
+    def precond_apply(self, db, dx):
+        check(hip().xh_solver_precond_apply(self.s, db.ptr, dx.ptr), "xh_solver_precond_apply")
+
+    def stats(self):
+        it = C.c_int32()
+        conv = np.zeros(self.nv, np.int32)
+        a = np.zeros(self.nv)
+        r = np.zeros(self.nv)
+        check(hip().xh_solver_stats(self.s, C.byref(it), _ptr(conv), _ptr(a), _ptr(r)),
+              "xh_solver_stats")
+        return it.value, conv, a, r
+
+    def history(self, max_checks=1000):
+        h = np.zeros(max_checks * self.nv)
+        nc = C.c_int32()
+        check(hip().xh_solver_history(self.s, _ptr(h), max_checks, C.byref(nc)),
+              "xh_solver_history")
+        return h[: nc.value * self.nv].reshape(nc.value, self.nv)
+
+    def info(self):
+        nl = C.c_int()
+        by = C.c_size_t()
+        check(hip().xh_solver_info(self.s, C.byref(nl), C.byref(by)), "xh_solver_info")
+        return nl.value, by.value
+
+    def __del__(self):
+        if getattr(self, "s", None) and _hip is not None:
+            _hip.xh_solver_destroy(self.s)
+            self.s = None
+
+
+class DeviceArray:
+    """A device buffer of float64/float32 values (xh_malloc)."""
+
+    def __init__(self, count, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        self.p = C.c_void_p()
+        check(hip().xh_malloc(C.byref(self.p), max(self.nbytes, 8)), "xh_malloc")
+
+    @property
+    def ptr(self):
+        return self.p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        check(hip().xh_memcpy_h2d(d.p, _ptr(a), a.nbytes), "xh_memcpy_h2d")
+        return d
+
+    def to_host(self):
+        a = np.empty(self.count, self.dtype)
+        check(hip().xh_memcpy_d2h(_ptr(a), self.p, self.nbytes), "xh_memcpy_d2h")
+        return a
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        check(hip().xh_memcpy_h2d(self.p, _ptr(a), a.nbytes), "xh_memcpy_h2d")
+
+    def __del__(self):
+        if getattr(self, "p", None) and _hip is not None:
+            _hip.xh_free(self.p)
+            self.p = None
+
+
+class Matrix:
+    def __init__(self, row, col, val, prec=0, index_mode=0, ncols=None):
+        n = len(row) - 1
+        self.row, self.col, self.val = (np.ascontiguousarray(row, np.uint32),
+                                        np.ascontiguousarray(col, np.uint32),
+                                        np.ascontiguousarray(val, np.float64))
+        self.m = C.c_void_p()
+        check(hip().xh_matrix_create(n, ncols if ncols is not None else n, len(col),
+                                     _ptr(self.row), _ptr(self.col), _ptr(self.val), prec,
+                                     index_mode, C.byref(self.m)), "xh_matrix_create")
+
+    def info(self):
+        nr, nc, nnz, pr, bits, panel = (C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(),
+                                        C.c_int(), C.c_int64())
+        by = C.c_size_t()
+        check(hip().xh_matrix_info(self.m, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(pr),
+                                   C.byref(bits), C.byref(panel), C.byref(by)), "xh_matrix_info")
+        return dict(nrows=nr.value, ncols=nc.value, nnz=nnz.value, prec=pr.value,
+                    index_bits=bits.value, panel_rows=panel.value, bytes=by.value)
+
+    def spmv(self, nv, dx, dy, accumulate=False):
+        check(hip().xh_spmv(self.m, nv, dx.ptr, dy.ptr, int(accumulate)), "xh_spmv")
+
+    def spmv_timed(self, nv, dx, dy, reps, accumulate=False):
+        ms = C.c_double()
+        check(hip().xh_spmv_timed(self.m, nv, dx.ptr, dy.ptr, int(accumulate), reps,
+                                  C.byref(ms)), "xh_spmv_timed")
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "m", None) and _hip is not None:
+            _hip.xh_matrix_destroy(self.m)
+            self.m = None
+
+
+def init(device=0):
+    check(hip().xh_init(device), "xh_init")
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    mem = C.c_double()
+    check(hip().xh_device_info(name, 256, C.byref(cu), C.byref(mem)), "xh_device_info")
+    return name.value.decode(), cu.value, mem.value
+
+
+def default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Chebyshev",
+                   poly_order=4, sweeps=1, rel_tol=1e-8, abs_tol=0.0, max_iters=50,
+                   coarse="Direct", mg_prec=0, hsgs_block_rows=64, graph=1, index_mode=0,
+                   mg_cycle=0, relax=0.99, fraction=0.3, coarse_fp64=1):
+    p = Params()
+    p.solver = role(method, max_iters=max_iters, abs_tol=abs_tol, rel_tol=rel_tol,
+                    conv_check=1)
+    p.precond = role(precond, max_iters=1) if precond else role(NONE)
+    sm = dict(max_iters=sweeps, relax_factor=relax, poly_order=poly_order,
+              spectrum_fraction=fraction)
+    p.pre = role(smoother, **sm)
+    p.post = role(smoother, **sm)
+    p.coarse = role(coarse, **sm)
+    p.mg_cycle = mg_cycle
+    p.mg_prec = mg_prec
+    p.hsgs_ncores = 1
+    p.hsgs_block_rows = hsgs_block_rows
+    p.coarse_fp64 = coarse_fp64
+    p.index_mode = index_mode
+    p.graph = graph
+    return p
