@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the XAMG solve phase on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the reference's 7-point
+cube Laplacian (examples/common/generator/cube.inl) on a 128^3 grid, 32 right-hand sides
+uniform in [-1,1] (SplitMix64, seed 1), x0 = 0, MergedPBiCGStab preconditioned by one
+V-cycle of the classical AMG hierarchy with degree-4 Chebyshev smoothing (1 pre + 1 post,
+Direct coarse solve), FP64 throughout, relative residual 1e-8 per RHS.
+
+One step = one complete solve to tolerance with the matrices and b resident in HBM
+(setup excluded, as examples/test/tst_solver.h:102-108).  value = ms per RHS.
+The SpMM roofline entry times the finest-level SpMM kernel (y = A x, nv = 32) with HIP
+events on the stream it is launched on.  The CPU baseline times the oracle (the CPU
+restatement of the reference solve) on one host core over a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "softx-d-21-00014_amd"))
+
+HBM_PEAK = 8.0e12  # MI355X published HBM3E bandwidth, bytes/s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--nv", type=int, default=32)
+    ap.add_argument("--poly-order", type=int, default=4)
+    ap.add_argument("--mg-prec", type=int, default=0)
+    ap.add_argument("--spmm-reps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-nv", type=int, default=4)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this build has no "
+              "multi-GPU path (see DESIGN.md for how the solve would shard); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import pyxamg as X
+
+    X.init(0)
+    N, nv = args.nx, args.nv
+    n = N ** 3
+    t0 = time.time()
+    row, col, val = X.cube(N)
+    amg = X.Amg(row, col, val)
+    levels = amg.export()
+    setup_s = time.time() - t0
+    H = X.Hier(levels)
+    prm = X.default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Chebyshev",
+                           poly_order=args.poly_order, sweeps=1, rel_tol=1e-8, abs_tol=0.0,
+                           max_iters=100, mg_prec=args.mg_prec, graph=1)
+    S = X.Solver(H, prm, nv)
+    b = X.rhs_uniform(1, n, nv)
+    db = X.DeviceArray.from_host(b)
+    dx = X.DeviceArray(n * nv)
+
+    for _ in range(args.warmup):
+        dx.zero()
+        S.solve_device(db, dx)
+    X.hip().xh_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dx.zero()
+        S.solve_device(db, dx)
+    X.hip().xh_synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    iters, conv, abs_res, rel_res = S.stats()
+
+    # true relative residual of the returned solution, with the GPU SpMM
+    A = X.Matrix(row, col, val, prec=0, index_mode=0)
+    dax = X.DeviceArray(n * nv)
+    A.spmv(nv, dx, dax)
+    r = (b - dax.to_host()).reshape(-1, nv)
+    true_rel = float(np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b.reshape(-1, nv), axis=0)))
+
+    # SpMM roofline: finest operator, y = A x, nv RHS, HIP events on the launch stream
+    info = A.info()
+    ms_spmm = A.spmv_timed(nv, dx, dax, args.spmm_reps)
+    idx_bytes = 2 if info["index_bits"] == 16 else 4
+    npanels = ((n + info["panel_rows"] - 1) // info["panel_rows"]) if info["panel_rows"] else 0
+    alg_bytes = info["nnz"] * (8 + idx_bytes) + (n + 1) * 4 + npanels * 4 + 2 * n * nv * 8
+    achieved = alg_bytes / (ms_spmm * 1e-3)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmm_cube128_nv32_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "ms per RHS to rel-res 1e-8 (128^3, nv=32); SpMM HBM GB/s vs roofline",
+        "value": round(ms_per_step / nv, 4),
+        "unit": "ms/RHS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.mg_prec == 0 else "f64/f32",
+        "data": "synthetic: reference cube generator (7-point FD Laplacian, Dirichlet), "
+                "RHS U[-1,1] from SplitMix64 seed 1, x0 = 0",
+        "config": {
+            "workload": f"cube {N}^3 7-point Laplacian, nv={nv}, MergedPBiCGStab + MultiGrid "
+                        f"V-cycle (Chebyshev order {args.poly_order} 1+1, Direct coarse), "
+                        f"rel-res 1e-8",
+            "rows": n, "nnz": int(len(col)), "nv": nv, "mg_levels": len(levels),
+            "mg_prec": args.mg_prec, "hipgraph_iteration": True,
+        },
+        "iters": iters,
+        "converged": int(conv.sum()),
+        "max_rel_res": float(rel_res.max()),
+        "true_rel_res": true_rel,
+        "setup_s": round(setup_s, 2),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved / 1e9, 1),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": traffic,
+            "kernel": f"k_spmv<double,{'uint16_t' if idx_bytes == 2 else 'uint32_t'},{nv},OP_SET>",
+            "algorithmic_bytes": int(alg_bytes),
+            "ms_per_launch": round(ms_spmm, 5),
+        },
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        ns = args.cpu_sample_nv
+        HO = O.Hier(levels)
+        op = O.params_from(prm)
+        op.solver.max_iters = 1          # one outer iteration: 2 V-cycles + 2 SpMM + dots
+        op.solver.rel_tol = 0.0
+        op.solver.abs_tol = 0.0
+        OS = O.SolverObj(HO, op)
+        bs = np.ascontiguousarray(b.reshape(-1, nv)[:, :ns]).reshape(-1)
+        c0 = time.perf_counter()
+        OS.solve(ns, bs)
+        c1 = time.perf_counter()
+        per_rhs_iter_ms = (c1 - c0) * 1e3 / ns
+        out["cpu_baseline"] = {
+            "value": round(per_rhs_iter_ms * max(iters, 1), 1),
+            "unit": "ms/RHS",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (single-threaded C++ restatement of the reference solve) on the "
+                      f"same {N}^3 hierarchy: 1 MergedPBiCGStab iteration for {ns} of the {nv} "
+                      f"RHS took {(c1 - c0):.2f} s; scaled by the GPU iteration count ({iters})",
+        }
+
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,11 @@ int xh_free(void *dptr);
 int xh_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int xh_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int xh_synchronize(void);
+/* dst[0:count] = src[0:count] (fp64) with the library's copy kernel, then synchronize;
+ * a kernel of known traffic (8*count read + 8*count written) for counter calibration */
+int xh_copy_device(double *dst, const double *src, size_t count);
+/* dptr[0:bytes] = 0 on the library stream, then synchronize (XAMG_vector_set_val_d(0)) */
+int xh_fill_zero(void *dptr, size_t bytes);
 
 /* CSR upload: replaces XAMG_csr_matrix_alloc/fill + XAMG_matrix_construct
  * (api/c/xamg_c_api.cpp:404-480).  Row/col are u32 (XAMG_LIMIT_TYPES_USAGE),

@@ -84,6 +84,12 @@ def lib():
         _lib.xo_solve_hist.restype = C.c_int
         _lib.xo_mg_apply.argtypes = [P(HierS), P(Params), C.c_int, vp, vp]
         _lib.xo_mg_apply.restype = C.c_int
+        _lib.xo_solver_create.argtypes = [P(HierS), P(Params), P(vp)]
+        _lib.xo_solver_create.restype = C.c_int
+        _lib.xo_solver_solve.argtypes = [vp, C.c_int, vp, vp, P(C.c_int32), vp, vp, vp]
+        _lib.xo_solver_solve.restype = C.c_int
+        _lib.xo_solver_free.argtypes = [vp]
+        _lib.xo_solver_free.restype = None
     return _lib
 
 
@@ -216,3 +222,33 @@ def mg_apply(hier, params, nv, b):
     if rc:
         raise RuntimeError(f"oracle mg_apply failed with {rc}")
     return x
+
+
+class SolverObj:
+    """oracle solver with setup separated from the solve (for timing the solve alone)"""
+
+    def __init__(self, hier, params):
+        self.hier = hier
+        self.params = params
+        self.s = C.c_void_p()
+        rc = lib().xo_solver_create(C.byref(hier.s), C.byref(params), C.byref(self.s))
+        if rc:
+            raise RuntimeError(f"oracle solver create failed with {rc}")
+
+    def solve(self, nv, b, x0=None):
+        n = self.hier.A[0].nrows
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n * nv) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        it = C.c_int32()
+        conv = np.zeros(nv, np.int32)
+        a = np.zeros(nv)
+        r = np.zeros(nv)
+        rc = lib().xo_solver_solve(self.s, nv, _p(b), _p(x), C.byref(it), _p(conv), _p(a), _p(r))
+        if rc:
+            raise RuntimeError(f"oracle solve failed with {rc}")
+        return dict(x=x, iters=it.value, conv=conv, abs_res=a, rel_res=r)
+
+    def __del__(self):
+        if getattr(self, "s", None) and _lib is not None:
+            _lib.xo_solver_free(self.s)
+            self.s = None

@@ -1550,14 +1550,69 @@ static std::unique_ptr<Solver<double>> make_precond(const xo_hier *H, const Para
     return make_smoother<double>(r, A0, H->min_eig[0], H->max_eig[0], P);
 }
 
-static int run_solve(const xo_hier *H, const xo_params *prm, int nv, const double *b,
-                     double *x, int32_t *iters, int32_t *converged, double *abs_res,
-                     double *rel_res, std::vector<double> *hist) {
+struct SolverSet {
+    Params P;
+    Csr<double> A0;
+    std::unique_ptr<MultiGrid> mg;
+    std::unique_ptr<Solver<double>> pre;
+    std::unique_ptr<Solver<double>> top;
+    Solver<double> *S = nullptr;
+    int dot_ncores = 1;
+};
+
+static int build_set(const xo_hier *H, const xo_params *prm, SolverSet &Z) {
     g_err = 0;
-    g_dot_ncores = std::max(1, prm->dot_ncores);
-    Params P = params_of(prm);
-    Csr<double> A0 = make_csr<double>(&H->A[0]);
-    const uint64_t n = A0.nrows;
+    Z.dot_ncores = std::max(1, prm->dot_ncores);
+    Z.P = params_of(prm);
+    Params &P = Z.P;
+    Z.A0 = make_csr<double>(&H->A[0]);
+    Solver<double> *precond = nullptr;
+    const int m = P.solver.method;
+    if (m == XO_PBICGSTAB || m == XO_MERGED_PBICGSTAB) {
+        Z.pre = make_precond(H, P, P.precond, &Z.A0, Z.mg);
+        precond = Z.mg ? static_cast<Solver<double> *>(Z.mg.get()) : Z.pre.get();
+        if (!precond)
+            return -4;
+    }
+    switch (m) {
+    case XO_PBICGSTAB:
+        Z.top.reset(new PBiCGStab());
+        break;
+    case XO_MERGED_PBICGSTAB:
+        Z.top.reset(new MergedPBiCGStab());
+        break;
+    case XO_BICGSTAB:
+        Z.top.reset(new BiCGStab());
+        break;
+    case XO_MERGED_BICGSTAB:
+        Z.top.reset(new MergedBiCGStab());
+        break;
+    case XO_MULTIGRID:
+        Z.mg.reset(new MultiGrid());
+        Z.mg->setup(H, P);
+        Z.mg->prm = P.solver;
+        break;
+    default:
+        Z.top = make_smoother<double>(P.solver, &Z.A0, H->min_eig[0], H->max_eig[0], P);
+        if (!Z.top)
+            return -4;
+        break;
+    }
+    Z.S = Z.top ? Z.top.get() : Z.mg.get();
+    Z.S->prm = P.solver;
+    if (auto *k = dynamic_cast<Krylov *>(Z.S)) {
+        k->A = &Z.A0;
+        k->precond = precond;
+    }
+    return g_err;
+}
+
+static int run_set(SolverSet &Z, int nv, const double *b, double *x, int32_t *iters,
+                   int32_t *converged, double *abs_res, double *rel_res,
+                   std::vector<double> *hist) {
+    g_err = 0;
+    g_dot_ncores = Z.dot_ncores;
+    const uint64_t n = Z.A0.nrows;
     Vec<double> xv, bv;
     xv.alloc(n, nv);
     bv.alloc(n, nv);
@@ -1565,51 +1620,10 @@ static int run_solve(const xo_hier *H, const xo_params *prm, int nv, const doubl
     std::memcpy(bv.p(), b, sizeof(double) * n * nv);
     xv.zero = false; /* user vectors are uploaded, never zero-flagged (blas_ext.inl:243-283) */
     bv.zero = false;
-
-    std::unique_ptr<MultiGrid> mg;
-    std::unique_ptr<Solver<double>> pre;
-    Solver<double> *precond = nullptr;
-    std::unique_ptr<Solver<double>> top;
-    const int m = P.solver.method;
-    if (m == XO_PBICGSTAB || m == XO_MERGED_PBICGSTAB) {
-        pre = make_precond(H, P, P.precond, &A0, mg);
-        precond = mg ? static_cast<Solver<double> *>(mg.get()) : pre.get();
-        if (!precond)
-            return -4;
-    }
-    switch (m) {
-    case XO_PBICGSTAB:
-        top.reset(new PBiCGStab());
-        break;
-    case XO_MERGED_PBICGSTAB:
-        top.reset(new MergedPBiCGStab());
-        break;
-    case XO_BICGSTAB:
-        top.reset(new BiCGStab());
-        break;
-    case XO_MERGED_BICGSTAB:
-        top.reset(new MergedBiCGStab());
-        break;
-    case XO_MULTIGRID: {
-        mg.reset(new MultiGrid());
-        mg->setup(H, P);
-        mg->prm = P.solver;
-        break;
-    }
-    default:
-        top = make_smoother<double>(P.solver, &A0, H->min_eig[0], H->max_eig[0], P);
-        if (!top)
-            return -4;
-        break;
-    }
-    Solver<double> *S = top ? top.get() : mg.get();
-    S->prm = P.solver;
-    if (auto *k = dynamic_cast<Krylov *>(S)) {
-        k->A = &A0;
-        k->precond = precond;
-    }
+    Solver<double> *S = Z.S;
     S->stats.hist = hist;
     S->solve(xv, bv);
+    S->stats.hist = nullptr;
     if (g_err)
         return g_err;
     std::memcpy(x, xv.p(), sizeof(double) * n * nv);
@@ -1624,6 +1638,16 @@ static int run_solve(const xo_hier *H, const xo_params *prm, int nv, const doubl
             rel_res[k] = k < (int)S->stats.rel_res.size() ? S->stats.rel_res[k] : 0.0;
     }
     return 0;
+}
+
+static int run_solve(const xo_hier *H, const xo_params *prm, int nv, const double *b,
+                     double *x, int32_t *iters, int32_t *converged, double *abs_res,
+                     double *rel_res, std::vector<double> *hist) {
+    SolverSet Z;
+    int rc = build_set(H, prm, Z);
+    if (rc)
+        return rc;
+    return run_set(Z, nv, b, x, iters, converged, abs_res, rel_res, hist);
 }
 
 } // namespace xo
@@ -1859,6 +1883,28 @@ int xo_solve_hist(const xo_hier *H, const xo_params *prm, int nv, const double *
         hist[i] = h[i];
     return rc;
 }
+
+struct xo_solver_s {
+    SolverSet Z;
+};
+
+int xo_solver_create(const xo_hier *H, const xo_params *prm, xo_solver *out) {
+    auto *s = new xo_solver_s();
+    int rc = build_set(H, prm, s->Z);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int xo_solver_solve(xo_solver S, int nv, const double *b, double *x, int32_t *iters,
+                    int32_t *converged, double *abs_res, double *rel_res) {
+    return run_set(S->Z, nv, b, x, iters, converged, abs_res, rel_res, nullptr);
+}
+
+void xo_solver_free(xo_solver S) { delete S; }
 
 int xo_mg_apply(const xo_hier *H, const xo_params *prm, int nv, const double *b, double *x) {
     g_err = 0;

@@ -128,6 +128,14 @@ int xo_solve_hist(const xo_hier *H, const xo_params *prm, int nv, const double *
                   int32_t *iters, int32_t *converged, double *abs_res, double *rel_res,
                   double *hist, int32_t max_hist);
 
+/* the same solve split into setup (hierarchy copies, smoother set-up, coarse inverse)
+ * and solve, so that the solve alone can be timed */
+typedef struct xo_solver_s *xo_solver;
+int xo_solver_create(const xo_hier *H, const xo_params *prm, xo_solver *out);
+int xo_solver_solve(xo_solver S, int nv, const double *b, double *x, int32_t *iters,
+                    int32_t *converged, double *abs_res, double *rel_res);
+void xo_solver_free(xo_solver S);
+
 /* one application of the MultiGrid preconditioner to b (x = M^{-1} b, x starts at 0) */
 int xo_mg_apply(const xo_hier *H, const xo_params *prm, int nv, const double *b, double *x);
 

@@ -1,14 +1,15 @@
 // Multi-RHS CSR SpMV (SpMM) for gfx950 with fused epilogues.
 //
 // Restates backend::csr_Ax_y / csr_Axpy (src/blas2/detail/csr_matvec.inl:33-260) on
-// NV-interleaved vectors x[i*NV+k]: one group of NV lanes per row, lane k owns RHS k,
-// so every nonzero reads one contiguous NV-wide row of x (a coalesced 8*NV-byte load
-// in fp64) and the row sum is accumulated in stored nonzero order exactly as the
-// reference (acc = acc + x*val, no FMA contraction; the file is built with
-// -ffp-contract=off).  Column indices are u32 or u16 panel-relative (col = base[row >>
-// shift] + u16), decoded in-kernel.  The epilogues fuse the level-1 work that follows
-// an SpMV in the reference solvers (residual, dot partials, Chebyshev steps,
-// mixed-precision prolongation, HSGS off-block residual).
+// NV-interleaved vectors x[i*NV+k].  A row is owned by L = NV/W lanes and each lane owns
+// W consecutive right-hand sides (W = 2 in fp64, 4 in fp32: one 16-byte load per lane),
+// so every nonzero reads one contiguous NV-wide row of x with 16-byte loads.  Each RHS
+// is accumulated in stored nonzero order exactly as the reference
+// (acc = acc + x*val, no FMA contraction: the file is built with -ffp-contract=off).
+// Column indices are u32 or u16 panel-relative (col = base[row >> shift] + u16), decoded
+// in-kernel.  The epilogues fuse the level-1 work that follows an SpMV in the reference
+// solvers (residual, dot partials, Chebyshev steps, mixed-precision prolongation, HSGS
+// off-block residual).
 //
 // Workgroups are dealt round-robin over the 8 XCDs; logical_block() renumbers them
 // so that each XCD walks one contiguous eighth of the rows, keeping the +-1 and +-nx
@@ -25,6 +26,21 @@ __device__ __forceinline__ int64_t logical_block() {
     return (int64_t)(p & 7u) * (int64_t)(G >> 3) + (int64_t)(p >> 3);
 }
 
+template <typename T, int W>
+struct alignas(sizeof(T) * W) Pack {
+    T v[W];
+};
+
+template <typename T, int W>
+__device__ __forceinline__ Pack<T, W> ld(const T *__restrict__ p) {
+    return *reinterpret_cast<const Pack<T, W> *>(p);
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void st(T *__restrict__ p, const Pack<T, W> &v) {
+    *reinterpret_cast<Pack<T, W> *>(p) = v;
+}
+
 template <typename IT>
 __device__ __forceinline__ uint32_t panel_base(const uint32_t *__restrict__ pb, int shift,
                                                int64_t i) {
@@ -34,48 +50,108 @@ __device__ __forceinline__ uint32_t panel_base(const uint32_t *__restrict__ pb, 
         return pb[i >> shift];
 }
 
-// acc + sum_j x[col_j]*val_j over row i, stored order, 4-way load batching
-template <typename VT, typename XT, typename IT, int NV>
-__device__ __forceinline__ VT row_acc(const uint32_t *__restrict__ row,
-                                      const IT *__restrict__ col, const VT *__restrict__ val,
-                                      uint32_t base, const XT *__restrict__ x, int64_t i, int k,
-                                      VT acc) {
-    uint32_t j = row[i];
-    const uint32_t e = row[i + 1];
-    for (; j + 4 <= e; j += 4) {
-        const uint32_t c0 = base + (uint32_t)col[j], c1 = base + (uint32_t)col[j + 1];
-        const uint32_t c2 = base + (uint32_t)col[j + 2], c3 = base + (uint32_t)col[j + 3];
-        const VT v0 = val[j], v1 = val[j + 1], v2 = val[j + 2], v3 = val[j + 3];
-        const VT x0 = (VT)x[(size_t)c0 * NV + k], x1 = (VT)x[(size_t)c1 * NV + k];
-        const VT x2 = (VT)x[(size_t)c2 * NV + k], x3 = (VT)x[(size_t)c3 * NV + k];
-        acc = acc + x0 * v0;
-        acc = acc + x1 * v1;
-        acc = acc + x2 * v2;
-        acc = acc + x3 * v3;
+// acc[w] + sum_j x[col_j][k0+w]*val_j over the nonzeros [j, e) of one row, stored order.
+// Nonzeros go in predicated batches of 8: all column/value loads of a batch are issued
+// together, then all x loads, so a row of up to 8 nonzeros costs two dependent memory
+// round trips.  Only existing entries are added (no +0 terms), so each RHS gets the
+// reference's sequential sum exactly.
+template <typename VT, typename IT, int NV, int W>
+__device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e, const IT *__restrict__ col,
+                                        const VT *__restrict__ val, uint32_t base,
+                                        const VT *__restrict__ x, int k0, VT (&acc)[W]) {
+    constexpr int BW = 8;
+    constexpr int L = NV / W; // lanes sharing this row
+    if constexpr (L >= 2) {
+        // The L lanes of the row load its column indices and values cooperatively
+        // (E entries per lane, one coalesced load each) and broadcast them across the
+        // group with cross-lane shuffles, instead of every lane loading every entry.
+        constexpr int E = (BW + L - 1) / L;
+        constexpr int CH = L * E; // entries per chunk (>= 8)
+        const int lane = threadIdx.x & 63;
+        const int g0 = lane - (lane % L);
+        const int gl = lane - g0;
+        for (; j < e; j += CH) {
+            uint32_t cl[E];
+            VT vl[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const uint32_t jj = j + q * L + gl;
+                const bool ok = jj < e;
+                cl[q] = ok ? base + (uint32_t)col[jj] : 0u;
+                vl[q] = ok ? val[jj] : VT(0);
+            }
+#pragma unroll
+            for (int b0 = 0; b0 < CH; b0 += BW) {
+                // uniform within the row's lane group, so every shuffle source is active
+                if (j + b0 >= e)
+                    break;
+                uint32_t c[BW];
+                VT v[BW];
+#pragma unroll
+                for (int q = 0; q < BW; ++q) {
+                    const int t = b0 + q; // entry index within the chunk
+                    c[q] = (uint32_t)__shfl((int)cl[t / L], g0 + (t % L), 64);
+                    v[q] = __shfl(vl[t / L], g0 + (t % L), 64);
+                }
+                Pack<VT, W> xv[BW];
+#pragma unroll
+                for (int q = 0; q < BW; ++q)
+                    if (j + b0 + q < e)
+                        xv[q] = ld<VT, W>(x + (size_t)c[q] * NV + k0);
+#pragma unroll
+                for (int q = 0; q < BW; ++q)
+                    if (j + b0 + q < e)
+#pragma unroll
+                        for (int w = 0; w < W; ++w)
+                            acc[w] = acc[w] + xv[q].v[w] * v[q];
+            }
+        }
+        return;
     }
-    for (; j < e; ++j) {
-        const uint32_t c = base + (uint32_t)col[j];
-        acc = acc + (VT)x[(size_t)c * NV + k] * val[j];
+    for (; j < e; j += BW) {
+        uint32_t c[BW];
+        VT v[BW];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            const bool ok = j + q < e;
+            c[q] = ok ? base + (uint32_t)col[j + q] : 0u;
+            v[q] = ok ? val[j + q] : VT(0);
+        }
+        Pack<VT, W> xv[BW];
+#pragma unroll
+        for (int q = 0; q < BW; ++q)
+            if (j + q < e)
+                xv[q] = ld<VT, W>(x + (size_t)c[q] * NV + k0);
+#pragma unroll
+        for (int q = 0; q < BW; ++q)
+            if (j + q < e)
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    acc[w] = acc[w] + xv[q].v[w] * v[q];
     }
-    return acc;
 }
 
-// sum over the lanes of a wave that own the same RHS, then over the 4 waves; one
-// partial per (dot, RHS) per workgroup, in a fixed order (deterministic)
-template <int NV, int ND>
-__device__ __forceinline__ void block_partials(double (&p)[ND], double *__restrict__ out,
+// Sum over the lanes that own the same RHS (lanes l and l + L, l + 2L, ... of a wave),
+// then over the 4 waves: one partial per (dot, RHS) per workgroup, fixed order.
+template <int NV, int W, int ND>
+__device__ __forceinline__ void block_partials(double (&p)[ND][W], double *__restrict__ out,
                                                int64_t lb) {
+    constexpr int L = NV / W;
     __shared__ double red[kBlock / 64][ND][NV];
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
-        for (int off = NV; off < 64; off <<= 1)
-            p[d] = p[d] + __shfl_xor(p[d], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane < NV)
+        for (int w = 0; w < W; ++w)
+#pragma unroll
+            for (int off = L; off < 64; off <<= 1)
+                p[d][w] = p[d][w] + __shfl_xor(p[d][w], off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane < L)
 #pragma unroll
         for (int d = 0; d < ND; ++d)
-            red[w][d][lane] = p[d];
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                red[wv][d][lane * W + w] = p[d][w];
     __syncthreads();
     if ((int)threadIdx.x < ND * NV) {
         const int d = threadIdx.x / NV, k = threadIdx.x % NV;
@@ -92,6 +168,11 @@ struct NDots {
     static constexpr int value = (OP == OP_DOT1 || OP == OP_RESID0) ? 1 : (OP == OP_DOT3 ? 3 : 0);
 };
 
+template <typename VT, int NV>
+struct Width {
+    static constexpr int value = (16 / (int)sizeof(VT)) < NV ? (16 / (int)sizeof(VT)) : NV;
+};
+
 template <typename VT, typename IT, int NV, int OP>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const uint32_t *__restrict__ row, const IT *__restrict__ col, const VT *__restrict__ val,
@@ -102,100 +183,180 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     int x_zero, int out_f32, double *__restrict__ partials, const int *__restrict__ done,
     int64_t blk_rows) {
     constexpr int ND = NDots<OP>::value;
+    constexpr int W = Width<VT, NV>::value;
+    constexpr int L = NV / W;        // lanes per row
+    constexpr int RPB = kBlock / L;  // rows per workgroup pass
+    using P = Pack<VT, W>;
     if (done != nullptr && *done)
         return;
-    constexpr int RPB = kBlock / NV;
     const int64_t lb = logical_block();
     const int64_t start = lb * chunk;
     const int64_t end = start + chunk < nrows ? start + chunk : nrows;
-    const int tr = threadIdx.x / NV, k = threadIdx.x % NV;
-    double part[ND > 0 ? ND : 1];
+    const int tr = threadIdx.x / L, k0 = (threadIdx.x % L) * W;
+    double part[ND > 0 ? ND : 1][W];
 #pragma unroll
     for (int d = 0; d < (ND > 0 ? ND : 1); ++d)
-        part[d] = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            part[d][w] = 0.0;
 
-    for (int64_t i = start + tr; i < end; i += RPB) {
-        const size_t o = (size_t)i * NV + k;
+    // row pointers of the next row are loaded one iteration ahead
+    int64_t i = start + tr;
+    uint32_t jb = 0, je = 0;
+    if (i < end) {
+        jb = row[i];
+        je = row[i + 1];
+    }
+    for (; i < end; i += RPB) {
+        const size_t o = (size_t)i * NV + k0;
         const uint32_t base = panel_base<IT>(pbase, shift, i);
-        if constexpr (OP == OP_SET) {
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
-            ((VT *)yv)[o] = acc;
-        } else if constexpr (OP == OP_ADD) {
-            VT *y = (VT *)yv;
-            y[o] = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, y[o]);
-        } else if constexpr (OP == OP_RESID) {
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
-            VT t = b[o] - acc;
-            if (out_f32)
-                ((float *)yv)[o] = (float)t;
-            else
-                ((VT *)yv)[o] = t;
+        uint32_t nb = 0, ne = 0;
+        if (i + RPB < end) {
+            nb = row[i + RPB];
+            ne = row[i + RPB + 1];
+        }
+        VT acc[W];
+        if constexpr (OP == OP_ADD) {
+            const P y0 = ld<VT, W>((const VT *)yv + o);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                acc[w] = y0.v[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                acc[w] = VT(0);
+        }
+        if constexpr (OP == OP_OFFBLOCK) {
+            // t = b - sum over off-block columns (old x), HSGS core blocks of blk_rows
+            const int64_t lo = (i / blk_rows) * blk_rows, hi = lo + blk_rows;
+            for (uint32_t j = jb; j < je; ++j) {
+                const int64_t cc = (int64_t)(base + (uint32_t)col[j]);
+                if (cc < lo || cc >= hi) {
+                    const P xv = ld<VT, W>(x + (size_t)cc * NV + k0);
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        acc[w] = acc[w] + xv.v[w] * val[j];
+                }
+            }
+        } else {
+            row_acc<VT, IT, NV, W>(jb, je, col, val, base, x, k0, acc);
+        }
+        if constexpr (OP == OP_SET || OP == OP_ADD) {
+            P yo;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                yo.v[w] = acc[w];
+            st<VT, W>((VT *)yv + o, yo);
+        } else if constexpr (OP == OP_RESID || OP == OP_OFFBLOCK) {
+            const P bv = ld<VT, W>(b + o);
+            if (OP == OP_RESID && out_f32) {
+                Pack<float, W> t;
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    t.v[w] = (float)(bv.v[w] - acc[w]);
+                st<float, W>((float *)yv + o, t);
+            } else {
+                P t;
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    t.v[w] = bv.v[w] - acc[w];
+                st<VT, W>((VT *)yv + o, t);
+            }
         } else if constexpr (OP == OP_RESID0) {
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
-            VT t = b[o] - acc;
-            ((VT *)yv)[o] = t;
-            out2[o] = t;
-            out3[o] = t;
-            part[0] = part[0] + (double)(t * t);
+            const P bv = ld<VT, W>(b + o);
+            P t;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                t.v[w] = bv.v[w] - acc[w];
+                part[0][w] = part[0][w] + (double)(t.v[w] * t.v[w]);
+            }
+            st<VT, W>((VT *)yv + o, t);
+            st<VT, W>(out2 + o, t);
+            st<VT, W>(out3 + o, t);
         } else if constexpr (OP == OP_DOT1) {
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
-            ((VT *)yv)[o] = acc;
-            part[0] = part[0] + (double)(acc * o1[o]);
+            const P ov = ld<VT, W>(o1 + o);
+            P yo;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                yo.v[w] = acc[w];
+                part[0][w] = part[0][w] + (double)(acc[w] * ov.v[w]);
+            }
+            st<VT, W>((VT *)yv + o, yo);
         } else if constexpr (OP == OP_DOT3) {
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
-            ((VT *)yv)[o] = acc;
-            const VT sv = o1[o];
-            part[0] = part[0] + (double)(acc * sv);
-            part[1] = part[1] + (double)(acc * acc);
-            part[2] = part[2] + (double)(sv * sv);
+            const P sv = ld<VT, W>(o1 + o);
+            P yo;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                yo.v[w] = acc[w];
+                part[0][w] = part[0][w] + (double)(acc[w] * sv.v[w]);
+                part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
+                part[2][w] = part[2][w] + (double)(sv.v[w] * sv.v[w]);
+            }
+            st<VT, W>((VT *)yv + o, yo);
         } else if constexpr (OP == OP_CHEB0) {
-            // r = b - A x ; r *= isd ; t1 = c*r ; then t1 *= isd (u) or x += t1*isd
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
+            // r = b - A x ; r *= isd ; t1 = c*r ; then t1 *= isd (u) or x (+)= t1*isd
             const VT di = isd[i];
-            VT rv = b[o] - acc;
-            rv = rv * di;
-            VT t1 = c * rv;
-            t1 = t1 * di;
+            const P bv = ld<VT, W>(b + o);
+            P rv, t1;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                rv.v[w] = bv.v[w] - acc[w];
+                rv.v[w] = rv.v[w] * di;
+                t1.v[w] = c * rv.v[w];
+                t1.v[w] = t1.v[w] * di;
+            }
             if (last) {
                 VT *xu = (VT *)xupdv;
-                xu[o] = x_zero ? t1 : t1 + xu[o];
+                if (!x_zero) {
+                    const P xo = ld<VT, W>(xu + o);
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        t1.v[w] = t1.v[w] + xo.v[w];
+                }
+                st<VT, W>(xu + o, t1);
             } else {
-                out2[o] = rv;
-                ((VT *)yv)[o] = t1;
+                st<VT, W>(out2 + o, rv);
+                st<VT, W>((VT *)yv + o, t1);
             }
         } else if constexpr (OP == OP_CHEB) {
             // t2 = (A u)*isd ; t1 = c*r + t2 ; then u' = t1*isd or x (+)= t1*isd
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
             const VT di = isd[i];
-            VT t2 = acc * di;
-            VT t1 = c * rr[o] + t2;
-            t1 = t1 * di;
+            const P r0 = ld<VT, W>(rr + o);
+            P t1;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const VT t2 = acc[w] * di;
+                t1.v[w] = c * r0.v[w] + t2;
+                t1.v[w] = t1.v[w] * di;
+            }
             if (last) {
                 VT *xu = (VT *)xupdv;
-                xu[o] = x_zero ? t1 : t1 + xu[o];
+                if (!x_zero) {
+                    const P xo = ld<VT, W>(xu + o);
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        t1.v[w] = t1.v[w] + xo.v[w];
+                }
+                st<VT, W>(xu + o, t1);
             } else {
-                ((VT *)yv)[o] = t1;
+                st<VT, W>((VT *)yv + o, t1);
             }
         } else if constexpr (OP == OP_PROLONG32) {
             // VT = float: temp32 = P xH ; x64 = (double)temp32 + x64
-            VT acc = row_acc<VT, VT, IT, NV>(row, col, val, base, x, i, k, VT(0));
             double *xd = (double *)yv;
-            xd[o] = x_zero ? (double)acc : (double)acc + xd[o];
-        } else if constexpr (OP == OP_OFFBLOCK) {
-            // t = b - sum over off-block columns (old x), HSGS core blocks of blk_rows
-            const int64_t blk = i / blk_rows;
-            const int64_t lo = blk * blk_rows, hi = lo + blk_rows;
-            VT acc = VT(0);
-            for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
-                const int64_t cc = (int64_t)(base + (uint32_t)col[j]);
-                if (cc < lo || cc >= hi)
-                    acc = acc + x[(size_t)cc * NV + k] * val[j];
-            }
-            ((VT *)yv)[o] = b[o] - acc;
+            Pack<double, W> xo;
+            if (!x_zero)
+                xo = ld<double, W>(xd + o);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                xo.v[w] = x_zero ? (double)acc[w] : (double)acc[w] + xo.v[w];
+            st<double, W>(xd + o, xo);
         }
+        jb = nb;
+        je = ne;
     }
     if constexpr (ND > 0)
-        block_partials<NV, ND>(part, partials, lb);
+        block_partials<NV, W, ND>(part, partials, lb);
 }
 
 struct Launch {
@@ -203,8 +364,7 @@ struct Launch {
     int64_t chunk;
 };
 
-Launch plan(int64_t nrows, int nv, int cap) {
-    const int rpb = kBlock / nv;
+Launch plan(int64_t nrows, int rpb, int cap) {
     int64_t need = (nrows + rpb - 1) / rpb;
     int64_t g = std::min<int64_t>(need, cap);
     g = std::max<int64_t>(8, (g + 7) / 8 * 8);
@@ -216,8 +376,10 @@ Launch plan(int64_t nrows, int nv, int cap) {
 template <typename VT, typename IT, int NV, int OP>
 int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_t s) {
     constexpr int ND = NDots<OP>::value;
+    constexpr int W = Width<VT, NV>::value;
+    constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
-    Launch L = plan(A.nrows, NV, cap);
+    Launch L = plan(A.nrows, RPB, cap);
     hipLaunchKernelGGL((k_spmv<VT, IT, NV, OP>), dim3(L.grid), dim3(kBlock), 0, s, A.row,
                        (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
                        A.nrows, L.chunk, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,

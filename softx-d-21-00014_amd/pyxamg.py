@@ -77,6 +77,7 @@ def _load():
     sig = {
         "xh_init": [C.c_int], "xh_finalize": [], "xh_malloc": [P(vp), C.c_size_t],
         "xh_free": [vp], "xh_memcpy_h2d": [vp, vp, C.c_size_t],
+        "xh_fill_zero": [vp, C.c_size_t], "xh_copy_device": [vp, vp, C.c_size_t],
         "xh_memcpy_d2h": [vp, vp, C.c_size_t], "xh_synchronize": [],
         "xh_device_info": [C.c_char_p, C.c_int, P(C.c_int), P(f64)],
         "xh_matrix_create": [i64, i64, i64, vp, vp, vp, C.c_int, C.c_int, P(vp)],
@@ -329,6 +330,9 @@ class DeviceArray:
         a = np.empty(self.count, self.dtype)
         check(hip().xh_memcpy_d2h(_ptr(a), self.p, self.nbytes), "xh_memcpy_d2h")
         return a
+
+    def zero(self):
+        check(hip().xh_fill_zero(self.p, self.nbytes), "xh_fill_zero")
 
     def upload(self, a):
         a = np.ascontiguousarray(a, self.dtype)
