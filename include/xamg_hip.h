@@ -76,7 +76,8 @@ int xh_fill_zero(void *dptr, size_t bytes);
  * values f64; prec 0 keeps f64, 1 stores f32 (matrix_block::assemble with reduced_prec,
  * src/primitives/matrix/matrix.h:79-93).  index_mode: 0 auto (u16 panel-relative
  * columns when every panel of >= 64 rows spans <= 65536 columns), 1 u32, 2 u16 (fails
- * if impossible).  Rows are re-ordered diagonal first, then ascending column. */
+ * if impossible).  Nonzeros keep their stored order: each row sum is accumulated in that
+ * order, as the reference does. */
 int xh_matrix_create(int64_t nrows, int64_t ncols, int64_t nnz, const uint32_t *row,
                      const uint32_t *col, const double *val, int prec, int index_mode,
                      xh_matrix *out);
@@ -123,7 +124,10 @@ typedef struct {
 
 /* Hierarchy (the output of hypre::get_full_hierarchy, hypre_hierarchy.inl:31-52):
  * level l holds A_l and, for l < nlevels-1, P_l (n_l x n_{l+1}) and R_l = P_l^T.
- * Arrays are copied. */
+ * Arrays are copied; stored nonzero order is kept.  The HSGS smoother accumulates the
+ * off-block part of a row in stored order, which equals the reference's block order
+ * when rows are stored diagonal first and then by ascending column (as the host AMG
+ * setup and the reference generators store them). */
 int xh_hier_create(int nlevels, xh_hier *out);
 int xh_hier_set_level(xh_hier H, int level, int64_t n, int64_t nnzA, const uint32_t *rowA,
                       const uint32_t *colA, const double *valA, int64_t nc, int64_t nnzP,

@@ -61,37 +61,6 @@ struct HostCsr {
     std::vector<double> val;
 };
 
-void sort_diag_first(HostCsr &m) {
-    for (int64_t i = 0; i < m.nrows; ++i) {
-        uint32_t b = m.row[i], e = m.row[i + 1];
-        std::vector<std::pair<uint32_t, double>> r;
-        r.reserve(e - b);
-        for (uint32_t j = b; j < e; ++j)
-            r.emplace_back(m.col[j], m.val[j]);
-        bool sorted = true;
-        for (size_t q = 0; q < r.size() && sorted; ++q) {
-            if (q == 0)
-                continue;
-            bool d0 = r[q - 1].first == (uint32_t)i, d1 = r[q].first == (uint32_t)i;
-            if (d1 || (!d0 && r[q - 1].first > r[q].first))
-                sorted = false;
-        }
-        if (sorted)
-            continue;
-        std::sort(r.begin(), r.end(), [i](const std::pair<uint32_t, double> &x,
-                                          const std::pair<uint32_t, double> &y) {
-            bool dx = x.first == (uint32_t)i, dy = y.first == (uint32_t)i;
-            if (dx != dy)
-                return dx;
-            return x.first < y.first;
-        });
-        for (uint32_t j = b; j < e; ++j) {
-            m.col[j] = r[j - b].first;
-            m.val[j] = r[j - b].second;
-        }
-    }
-}
-
 // largest panel shift (rows per panel = 2^s, s >= 6) whose every panel spans at most
 // 65536 columns; -1 when none does
 int find_panel_shift(const HostCsr &m) {
@@ -823,7 +792,6 @@ int xh_matrix_create(int64_t nrows, int64_t ncols, int64_t nnz, const uint32_t *
     for (int64_t j = 0; j < nnz; ++j)
         if ((int64_t)h.col[j] >= ncols)
             return fail(XH_EINVAL, "xh_matrix_create: column index out of range");
-    sort_diag_first(h);
     auto *m = new xh_matrix_s();
     rc = upload_mat(h, prec, index_mode, m->m);
     if (rc) {
@@ -950,7 +918,6 @@ int xh_hier_set_level(xh_hier H, int level, int64_t n, int64_t nnzA, const uint3
     int rc = copy_csr(H->A[level], n, n, nnzA, rowA, colA, valA);
     if (rc)
         return rc;
-    sort_diag_first(H->A[level]);
     if (level + 1 < H->nlevels) {
         if ((rc = copy_csr(H->P[level], n, nc, nnzP, rowP, colP, valP)))
             return rc;

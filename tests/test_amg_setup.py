@@ -1,0 +1,67 @@
+"""Properties of the host classical AMG setup (the hierarchy the solve consumes)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+
+def mat(t):
+    # copies: scipy keeps a float64 data array without copying and may sort indices in
+    # place, which would permute the values of the exported level
+    n, m, row, col, val = t
+    return sp.csr_matrix((val.copy(), col.astype(np.int64), row.astype(np.int64)), shape=(n, m))
+
+
+@pytest.mark.parametrize("N", [10, 20])
+def test_hierarchy_properties(X, N):
+    row, col, val = X.cube(N)
+    amg = X.Amg(row, col, val, coarse_size=50)
+    lv = amg.export()
+    assert len(lv) >= 2
+    sizes = [d["A"][0] for d in lv]
+    assert all(a > b for a, b in zip(sizes, sizes[1:]))
+    assert sizes[-1] <= 50 or len(lv) == 25
+    for l, d in enumerate(lv):
+        n, _, r, c, v = d["A"]
+        # rows stored diagonal first, then ascending columns
+        for i in range(0, n, max(1, n // 40)):
+            cs = c[r[i]:r[i + 1]].astype(np.int64)
+            assert cs[0] == i and np.all(np.diff(cs[1:]) > 0)
+        if l + 1 == len(lv):
+            break
+        A, P, R = mat(d["A"]), mat(d["P"]), mat(d["R"])
+        assert (abs(R - P.T) > 0).nnz == 0  # R = P^T exactly
+        Ac = mat(lv[l + 1]["A"])
+        G = R @ A @ P
+        assert abs(Ac - G).max() <= 1e-10 * abs(G).max()  # Galerkin
+        # interpolation: C points are unit rows, F rows have at most Pmax entries
+        nnz_row = np.diff(d["P"][2])
+        assert nnz_row.max() <= 4
+        cpts = np.where((nnz_row == 1) & np.isclose(P.max(axis=1).toarray().ravel(), 1.0))[0]
+        assert len(cpts) >= d["P"][1]
+        # the coarse operator of an M-matrix stays symmetric positive definite
+        assert abs(Ac - Ac.T).max() <= 1e-10 * abs(Ac).max()
+        assert (Ac.diagonal() > 0).all()
+
+
+def test_eigenvalue_estimate_of_cube(X):
+    # lambda_max of D^-1/2 A D^-1/2 for the cube operator: power iteration from below,
+    # bounded by the Gershgorin bound 2 and close to the closed form
+    N = 12
+    row, col, val = X.cube(N)
+    amg = X.Amg(row, col, val, max_levels=1, eig_iters=60)
+    mn, mx = amg.eigs(0)
+    A = sp.csr_matrix((val, col, row), shape=(N ** 3, N ** 3)).toarray()
+    d = np.diag(A)
+    ev = np.linalg.eigvalsh(A / np.sqrt(np.outer(d, d)))
+    assert mn == 0.0 and ev[-1] * 0.95 < mx <= ev[-1] + 1e-12
+
+
+def test_setup_is_deterministic(X):
+    row, col, val = X.cube(14)
+    a = X.Amg(row, col, val).export()
+    b = X.Amg(row, col, val).export()
+    for da, db in zip(a, b):
+        for k in ("A", "P", "R"):
+            if k in da:
+                for u, v in zip(da[k][2:], db[k][2:]):
+                    assert np.array_equal(u, v)
