@@ -54,7 +54,10 @@ def main():
     n = N ** 3
     t0 = time.time()
     row, col, val = X.cube(N)
-    amg = X.Amg(row, col, val)
+    # MultiGrid keys of the reference's own test configurations
+    # (examples/test/yaml/single_node/*.yaml): mg_agg_num_levels 2, mg_num_paths 2,
+    # mg_coarse_matrix_size 500
+    amg = X.Amg(row, col, val, agg_num_levels=2, num_paths=2, coarse_size=500)
     levels = amg.export()
     setup_s = time.time() - t0
     H = X.Hier(levels)
@@ -115,9 +118,11 @@ def main():
                 "RHS U[-1,1] from SplitMix64 seed 1, x0 = 0",
         "config": {
             "workload": f"cube {N}^3 7-point Laplacian, nv={nv}, MergedPBiCGStab + MultiGrid "
-                        f"V-cycle (Chebyshev order {args.poly_order} 1+1, Direct coarse), "
+                        f"V-cycle (Chebyshev order {args.poly_order} 1+1, Direct coarse; "
+                        f"aggressive coarsening 2 levels, 2 paths, coarse size 500), "
                         f"rel-res 1e-8",
             "rows": n, "nnz": int(len(col)), "nv": nv, "mg_levels": len(levels),
+            "operator_complexity": round(sum(len(d["A"][3]) for d in levels) / len(col), 3),
             "mg_prec": args.mg_prec, "hipgraph_iteration": True,
         },
         "iters": iters,

@@ -29,6 +29,10 @@ typedef struct {
     float trunc_factor; /* mg_trunc_factor 0.25 */
     int32_t pmax;       /* mg_Pmax_elements 4 */
     int32_t second_pass;/* Ruge-Stueben second pass (0) */
+    int32_t agg_num_levels; /* mg_agg_num_levels: levels coarsened aggressively (10) */
+    int32_t num_paths;  /* mg_num_paths (3) */
+    float agg_trunc_factor; /* mg_agg_trunc_factor 0.25 */
+    int32_t agg_pmax;   /* mg_agg_Pmax_elements 4 */
     int32_t eig_iters;  /* power iterations for Chebyshev bounds (10) */
     uint64_t eig_seed;
 } xamg_amg_params;

@@ -94,6 +94,28 @@ int xh_spmv(xh_matrix A, int nv, const void *dx, void *dy, int accumulate);
  * that stream: average milliseconds per launch. */
 int xh_spmv_timed(xh_matrix A, int nv, const void *dx, void *dy, int accumulate, int reps,
                   double *ms_per_launch);
+/* Row schedule of the SpMV launches (no counterpart in the reference, which walks rows
+ * in order on each core: csr_matvec.inl:33-260).  A square matrix whose rows couple
+ * mostly to rows up to `fold` away (a structured-grid operator: fold = nx*ny) is
+ * processed as an (nrows/fold) x fold array in column strips, one or a few per XCD,
+ * so each x row is reused from L2; the per-row arithmetic, and so y, is unchanged.
+ * xh_matrix_create detects fold (the most frequent largest positive column offset, if
+ * it is the largest one of at least a quarter of the rows and >= 256).
+ * xh_matrix_fold: set >= 0 overrides it (0: plain row order), set < 0 only queries. */
+int xh_matrix_fold(xh_matrix A, int64_t set, int64_t *fold);
+/* the detection rule above on a host CSR matrix */
+int xh_csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_t *col,
+                int64_t *fold);
+/* The launch shape of an SpMV over nrows rows: workgroups, tiles, tiles per workgroup,
+ * and (rows != NULL) the [begin, end) row range of every tile, 2*ntiles entries, in
+ * processing order.  dots != 0: a launch that writes dot-product partials. */
+int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv, int dots,
+                     int64_t *grid, int64_t *ntiles, int64_t *tpw, int64_t *rows);
+/* Launch-shape knobs: "spmv_strip_budget" (bytes, default 8 MiB), "spmv_tile_passes"
+ * (default 8), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
+ * when 2*fold*nv*sizeof(value) exceeds it). */
+int xh_set_option(const char *key, int64_t value);
+
 /* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
 int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);
 

@@ -29,8 +29,69 @@ struct Mat {
     void *val = nullptr;       // nnz of double or float
     void *inv_diag = nullptr;      // nrows, prec of the matrix
     void *inv_sqrt_diag = nullptr; // nrows, prec of the matrix
+    int64_t fold = 0;          // dominant long-range column offset (0: none), see Sched
     size_t bytes = 0;          // device bytes owned
 };
+
+// Row schedule of an SpMV launch.  The rows are viewed as a (na x fold) array (row
+// r = a*fold + b) cut into column strips of `strip` rows and tiles of `tile` contiguous
+// rows; tiles are numbered strip-major, then by a, then by position in the strip:
+//   tile k -> strip k / (na*tps), a = (k / tps) % na, t = k % tps,
+//   rows [a*fold + strip*s + t*tile, ... + tile) clipped to the strip and to nrows.
+// A matrix whose rows couple mostly to rows +-fold away (a structured-grid operator,
+// fold = nx*ny) then reuses each x row within one strip a few tiles after its first
+// use, inside one XCD's L2, instead of fold rows later.  fold = strip = nrows, na = 1
+// is plain contiguous order.  Workgroup w processes tiles [w*tpw, (w+1)*tpw).
+struct Sched {
+    int64_t nrows = 0, fold = 0, strip = 0, na = 1, ntiles = 0;
+    int32_t tile = 0, tps = 0, tpw = 1;
+};
+
+// rows [b, e) of tile k (empty when the tile lies past its strip or the matrix)
+__host__ __device__ inline void tile_rows(const Sched &s, int64_t k, int64_t &b, int64_t &e) {
+    const int64_t per_strip = s.na * s.tps;
+    const int64_t st = k / per_strip;
+    const int64_t rem = k - st * per_strip;
+    const int64_t a = rem / s.tps;
+    const int64_t t = rem - a * s.tps;
+    const int64_t s0 = st * s.strip;
+    const int64_t s1 = s0 + s.strip < s.fold ? s0 + s.strip : s.fold;
+    const int64_t lo = s0 + t * s.tile;
+    const int64_t hi = lo + s.tile < s1 ? lo + s.tile : s1;
+    b = a * s.fold + lo;
+    e = a * s.fold + hi;
+    if (b > s.nrows)
+        b = s.nrows;
+    if (e > s.nrows)
+        e = s.nrows;
+    if (e < b)
+        e = b;
+}
+
+struct Launch {
+    int grid = 0;
+    Sched sc;
+};
+
+// Tile schedule and grid of one SpMV launch over n rows with nnz nonzeros and fold F
+// (0: none): rpb rows per workgroup pass, at most cap workgroups, row_bytes bytes of x
+// per row.
+Launch spmv_plan(int64_t n, int64_t nnz, int64_t F, int rpb, int cap, int64_t row_bytes);
+
+// dominant long-range column offset of a square CSR matrix: the most frequent largest
+// positive (col - row) over the rows, if it is the largest one of at least a quarter
+// of the rows and at least `min_fold`; else 0
+int64_t csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_t *col,
+                 int64_t min_fold);
+
+// Launch-shape knobs (xh_set_option)
+struct Tuning {
+    int64_t strip_budget = 8 << 20; // bytes of x+y+A touched between reuses of an x row
+    int tile_passes = 8;            // tile rows = tile_passes * rows per workgroup pass
+    int fold_enable = 1;            // 0: always contiguous order
+    int64_t fold_min_bytes = 2 << 20; // fold only when 2*fold*nv*sizeof(value) exceeds this
+};
+extern Tuning g_tune;
 
 // Scalar state of a Krylov solve, resident in device memory.  One slot per RHS.
 struct SolverState {

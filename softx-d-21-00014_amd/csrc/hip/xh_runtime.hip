@@ -23,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace xh {
@@ -82,6 +83,31 @@ int find_panel_shift(const HostCsr &m) {
     }
     return -1;
 }
+
+} // namespace
+
+int64_t csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_t *col,
+                 int64_t min_fold) {
+    if (nrows != ncols || nrows <= 0)
+        return 0;
+    std::unordered_map<int64_t, int64_t> cnt;
+    for (int64_t i = 0; i < nrows; ++i) {
+        int64_t mx = 0;
+        for (uint32_t j = row[i]; j < row[i + 1]; ++j)
+            mx = std::max<int64_t>(mx, (int64_t)col[j] - i);
+        if (mx > 0)
+            ++cnt[mx];
+    }
+    int64_t best = 0, bc = 0;
+    for (const auto &kv : cnt)
+        if (kv.second > bc || (kv.second == bc && kv.first > best)) {
+            best = kv.first;
+            bc = kv.second;
+        }
+    return (bc * 4 >= nrows && best >= min_fold) ? best : 0;
+}
+
+namespace {
 
 template <typename T>
 int dmalloc(T **p, size_t count, size_t &acc) {
@@ -185,6 +211,8 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
         M.inv_diag = a;
         M.inv_sqrt_diag = b;
     }
+    if (h.nrows == h.ncols)
+        M.fold = csr_fold(h.nrows, h.ncols, h.row.data(), h.col.data(), 256);
     M.bytes = acc;
     return 0;
 }
@@ -1227,5 +1255,65 @@ extern "C" int xh_copy_device(double *dst, const double *src, size_t count) {
     copy_vec((int64_t)count, F64, src, dst, nullptr, g_stream);
     XH_HIP(hipGetLastError());
     XH_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+extern "C" int xh_set_option(const char *key, int64_t value) {
+    if (!key)
+        return fail(XH_EINVAL, "xh_set_option: null key");
+    const std::string k(key);
+    if (k == "spmv_strip_budget" && value > 0)
+        g_tune.strip_budget = value;
+    else if (k == "spmv_tile_passes" && value >= 1 && value <= 1024)
+        g_tune.tile_passes = (int)value;
+    else if (k == "spmv_fold_enable" && (value == 0 || value == 1))
+        g_tune.fold_enable = (int)value;
+    else if (k == "spmv_fold_min_bytes" && value >= 0)
+        g_tune.fold_min_bytes = value;
+    else
+        return fail(XH_EINVAL, "xh_set_option: unknown key or value out of range: " + k);
+    return 0;
+}
+
+extern "C" int xh_csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_t *col,
+                           int64_t *fold) {
+    if (!row || !fold || (nrows > 0 && row[nrows] > 0 && !col))
+        return fail(XH_EINVAL, "xh_csr_fold: null pointer");
+    *fold = csr_fold(nrows, ncols, row, col, 256);
+    return 0;
+}
+
+extern "C" int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
+                                int dots, int64_t *grid, int64_t *ntiles, int64_t *tpw,
+                                int64_t *rows) {
+    if (nrows < 0 || nnz < 0 || fold < 0 || (prec != F64 && prec != F32) || !nv_ok(nv))
+        return fail(XH_EINVAL, "xh_spmv_schedule: bad argument");
+    const int vb = prec == F64 ? 8 : 4;
+    const int w = std::min(16 / vb, nv);
+    const int rpb = kBlock / (nv / w);
+    const Launch L = spmv_plan(nrows, nnz, fold, rpb, dots ? kMaxPartials : 16384,
+                               (int64_t)nv * vb);
+    if (grid)
+        *grid = L.grid;
+    if (ntiles)
+        *ntiles = L.sc.ntiles;
+    if (tpw)
+        *tpw = L.sc.tpw;
+    if (rows)
+        for (int64_t k = 0; k < L.sc.ntiles; ++k)
+            tile_rows(L.sc, k, rows[2 * k], rows[2 * k + 1]);
+    return 0;
+}
+
+extern "C" int xh_matrix_fold(xh_matrix A, int64_t set, int64_t *fold) {
+    if (!A)
+        return fail(XH_EINVAL, "xh_matrix_fold: null matrix");
+    if (set >= 0) {
+        if (set > 0 && (A->m.nrows != A->m.ncols || set >= A->m.nrows))
+            return fail(XH_EINVAL, "xh_matrix_fold: fold must be below nrows of a square matrix");
+        A->m.fold = set;
+    }
+    if (fold)
+        *fold = A->m.fold;
     return 0;
 }

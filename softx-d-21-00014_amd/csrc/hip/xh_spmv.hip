@@ -12,13 +12,18 @@
 // off-block residual).
 //
 // Workgroups are dealt round-robin over the 8 XCDs; logical_block() renumbers them
-// so that each XCD walks one contiguous eighth of the rows, keeping the +-1 and +-nx
-// neighbours of x in that XCD's L2.
+// so that each XCD walks one contiguous eighth of the tile schedule (Sched): in plain
+// order one eighth of the rows, in folded order one column strip (or a few), so that
+// the +-1, +-nx and +-fold neighbours of x are reused from that XCD's L2.
 #include "xh_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace xh {
+
+Tuning g_tune;
+
 namespace {
 
 __device__ __forceinline__ int64_t logical_block() {
@@ -36,9 +41,16 @@ __device__ __forceinline__ Pack<T, W> ld(const T *__restrict__ p) {
     return *reinterpret_cast<const Pack<T, W> *>(p);
 }
 
+// Results are streamed out with nontemporal stores, so they do not evict the x rows
+// that later rows of the launch will read again from L2.
 template <typename T, int W>
 __device__ __forceinline__ void st(T *__restrict__ p, const Pack<T, W> &v) {
-    *reinterpret_cast<Pack<T, W> *>(p) = v;
+    typedef T vec __attribute__((ext_vector_type(W)));
+    vec t;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        t[w] = v.v[w];
+    __builtin_nontemporal_store(t, reinterpret_cast<vec *>(p));
 }
 
 template <typename IT>
@@ -50,84 +62,162 @@ __device__ __forceinline__ uint32_t panel_base(const uint32_t *__restrict__ pb, 
         return pb[i >> shift];
 }
 
-// acc[w] + sum_j x[col_j][k0+w]*val_j over the nonzeros [j, e) of one row, stored order.
-// Nonzeros go in predicated batches of 8: all column/value loads of a batch are issued
-// together, then all x loads, so a row of up to 8 nonzeros costs two dependent memory
-// round trips.  Only existing entries are added (no +0 terms), so each RHS gets the
-// reference's sequential sum exactly.
-template <typename VT, typename IT, int NV, int W>
-__device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e, const IT *__restrict__ col,
-                                        const VT *__restrict__ val, uint32_t base,
-                                        const VT *__restrict__ x, int k0, VT (&acc)[W]) {
-    constexpr int BW = 8;
-    constexpr int L = NV / W; // lanes sharing this row
-    if constexpr (L >= 2) {
-        // The L lanes of the row load its column indices and values cooperatively
-        // (E entries per lane, one coalesced load each) and broadcast them across the
-        // group with cross-lane shuffles, instead of every lane loading every entry.
-        constexpr int E = (BW + L - 1) / L;
-        constexpr int CH = L * E; // entries per chunk (>= 8)
+// The nonzeros of a row go in chunks.  With L >= 2 lanes per row, the L lanes load a
+// chunk's column indices and values cooperatively (E entries per lane, one coalesced load
+// each) and broadcast them across the group with cross-lane shuffles; with one lane per
+// row the lane loads 8 entries itself.  The first chunk of the next row is loaded one
+// pass ahead (together with its row pointers and panel base), so a row of up to CH
+// nonzeros costs one exposed memory round trip: its x loads.  Only existing entries are
+// added (no +0 terms), so each RHS gets the reference's sequential sum exactly.
+// Broadcast of lane S of each L-lane group (L consecutive lanes, L | 64) to the whole
+// group, S a compile-time constant.  DPP (a VALU operand modifier, no LDS traffic):
+// quad_perm for L = 2, 4; row_newbcast (gfx90a+: every lane of a 16-lane row reads
+// lane S of that row) for L = 16, and two of them merged for L = 8.  L = 32 goes
+// through ds_bpermute.
+template <int L, int S>
+__device__ __forceinline__ int bcast_i32(int v) {
+    static_assert(S >= 0 && S < L, "source lane outside the group");
+    if constexpr (L == 2) {
+        constexpr int q = S | (S << 2) | ((2 + S) << 4) | ((2 + S) << 6);
+        return __builtin_amdgcn_update_dpp(0, v, q, 0xF, 0xF, false);
+    } else if constexpr (L == 4) {
+        constexpr int q = S | (S << 2) | (S << 4) | (S << 6);
+        return __builtin_amdgcn_update_dpp(0, v, q, 0xF, 0xF, false);
+    } else if constexpr (L == 16) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xF, 0xF, false);
+    } else if constexpr (L == 8) {
+        const int lo = __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, v, 0x158 + S, 0xF, 0xF, false);
+        return (threadIdx.x & 8) ? hi : lo;
+    } else {
         const int lane = threadIdx.x & 63;
-        const int g0 = lane - (lane % L);
-        const int gl = lane - g0;
-        for (; j < e; j += CH) {
-            uint32_t cl[E];
-            VT vl[E];
-#pragma unroll
-            for (int q = 0; q < E; ++q) {
-                const uint32_t jj = j + q * L + gl;
-                const bool ok = jj < e;
-                cl[q] = ok ? base + (uint32_t)col[jj] : 0u;
-                vl[q] = ok ? val[jj] : VT(0);
-            }
-#pragma unroll
-            for (int b0 = 0; b0 < CH; b0 += BW) {
-                // uniform within the row's lane group, so every shuffle source is active
-                if (j + b0 >= e)
-                    break;
-                uint32_t c[BW];
-                VT v[BW];
-#pragma unroll
-                for (int q = 0; q < BW; ++q) {
-                    const int t = b0 + q; // entry index within the chunk
-                    c[q] = (uint32_t)__shfl((int)cl[t / L], g0 + (t % L), 64);
-                    v[q] = __shfl(vl[t / L], g0 + (t % L), 64);
-                }
-                Pack<VT, W> xv[BW];
-#pragma unroll
-                for (int q = 0; q < BW; ++q)
-                    if (j + b0 + q < e)
-                        xv[q] = ld<VT, W>(x + (size_t)c[q] * NV + k0);
-#pragma unroll
-                for (int q = 0; q < BW; ++q)
-                    if (j + b0 + q < e)
-#pragma unroll
-                        for (int w = 0; w < W; ++w)
-                            acc[w] = acc[w] + xv[q].v[w] * v[q];
-            }
-        }
-        return;
+        return __shfl(v, lane - (lane % L) + S, 64);
     }
-    for (; j < e; j += BW) {
-        uint32_t c[BW];
-        VT v[BW];
+}
+
+template <int L, int S, typename T>
+__device__ __forceinline__ T bcast(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, bcast_i32<L, S>(__builtin_bit_cast(int, v)));
+    } else {
+        static_assert(sizeof(T) == 8, "4- or 8-byte values");
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const int lo = bcast_i32<L, S>((int)(unsigned)u);
+        const int hi = bcast_i32<L, S>((int)(unsigned)(u >> 32));
+        return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+
+// compile-time loop: f(std::integral_constant<int, 0>), ..., f(<N-1>)
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>());
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <typename VT, int NV, int W>
+struct Chunk {
+    static constexpr int BW = 8;
+    static constexpr int L = NV / W;                        // lanes sharing the row
+    static constexpr int E = L >= 2 ? (BW + L - 1) / L : BW; // entries per lane
+    static constexpr int CH = L >= 2 ? L * E : BW;           // entries per chunk (>= 8)
+    uint32_t c[E]; // raw column index (panel-relative for u16)
+    VT v[E];
+};
+
+template <typename VT, typename IT, int NV, int W>
+__device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
+                                           const VT *__restrict__ val, Chunk<VT, NV, W> &ch) {
+    using C = Chunk<VT, NV, W>;
+    const int gl = C::L >= 2 ? (int)(threadIdx.x & 63) % C::L : 0;
+#pragma unroll
+    for (int q = 0; q < C::E; ++q) {
+        const uint32_t jj = j + (C::L >= 2 ? (uint32_t)(q * C::L + gl) : (uint32_t)q);
+        const bool ok = jj < e;
+        ch.c[q] = ok ? (uint32_t)col[jj] : 0u;
+        ch.v[q] = ok ? val[jj] : VT(0);
+    }
+}
+
+// acc[w] + sum x[base + c][k0 + w] * v over the entries [j, min(j + CH, e)) of `ch`.
+// The x loads of a batch are unconditional (a slot past the row end reloads the batch's
+// first entry, which exists) and the products of such slots are dropped by a select, so
+// no load sits behind its own branch and the loads of a batch stay in flight together.
+template <typename VT, int NV, int W>
+__device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chunk<VT, NV, W> &ch,
+                                              uint32_t base, const VT *__restrict__ x, int k0,
+                                              VT (&acc)[W]) {
+    using C = Chunk<VT, NV, W>;
+    constexpr int BW = C::BW, L = C::L;
+    if constexpr (L >= 2) {
+#pragma unroll
+        for (int b0 = 0; b0 < C::CH; b0 += BW) {
+            // uniform within the row's lane group, so every broadcast source is active
+            if (j + b0 >= e)
+                break;
+            uint32_t c[BW];
+            static_for<0, BW>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                // entry b0 + q of the chunk sits in lane (b0 + q) % L, slot (b0 + q) / L
+                if (b0 == 0)
+                    c[q] = (uint32_t)bcast<L, q % L>((int)ch.c[q / L]);
+                else
+                    c[q] = (uint32_t)bcast<L, (q + BW) % L>((int)ch.c[(q + BW) / L]);
+            });
+            Pack<VT, W> xv[BW];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) {
+                const uint32_t cq = (j + b0 + q < e) ? c[q] : c[0];
+                xv[q] = ld<VT, W>(x + (size_t)(base + cq) * NV + k0);
+            }
+            static_for<0, BW>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const VT vq = b0 == 0 ? bcast<L, q % L>(ch.v[q / L])
+                                      : bcast<L, (q + BW) % L>(ch.v[(q + BW) / L]);
+                const bool ok = j + b0 + q < e;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const VT t = acc[w] + xv[q].v[w] * vq;
+                    acc[w] = ok ? t : acc[w];
+                }
+            });
+        }
+    } else {
+        if (j >= e)
+            return;
+        Pack<VT, W> xv[BW];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            const uint32_t cq = (j + q < e) ? ch.c[q] : ch.c[0];
+            xv[q] = ld<VT, W>(x + (size_t)(base + cq) * NV + k0);
+        }
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             const bool ok = j + q < e;
-            c[q] = ok ? base + (uint32_t)col[j + q] : 0u;
-            v[q] = ok ? val[j + q] : VT(0);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const VT t = acc[w] + xv[q].v[w] * ch.v[q];
+                acc[w] = ok ? t : acc[w];
+            }
         }
-        Pack<VT, W> xv[BW];
-#pragma unroll
-        for (int q = 0; q < BW; ++q)
-            if (j + q < e)
-                xv[q] = ld<VT, W>(x + (size_t)c[q] * NV + k0);
-#pragma unroll
-        for (int q = 0; q < BW; ++q)
-            if (j + q < e)
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    acc[w] = acc[w] + xv[q].v[w] * v[q];
+    }
+}
+
+// the whole row [j, e), its first chunk already in `first`
+template <typename VT, typename IT, int NV, int W>
+__device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
+                                        const Chunk<VT, NV, W> &first,
+                                        const IT *__restrict__ col, const VT *__restrict__ val,
+                                        uint32_t base, const VT *__restrict__ x, int k0,
+                                        VT (&acc)[W]) {
+    using C = Chunk<VT, NV, W>;
+    consume_chunk<VT, NV, W>(j, e, first, base, x, k0, acc);
+    for (j += C::CH; j < e; j += C::CH) {
+        C ch;
+        load_chunk<VT, IT, NV, W>(j, e, col, val, ch);
+        consume_chunk<VT, NV, W>(j, e, ch, base, x, k0, acc);
     }
 }
 
@@ -176,7 +266,7 @@ struct Width {
 template <typename VT, typename IT, int NV, int OP>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const uint32_t *__restrict__ row, const IT *__restrict__ col, const VT *__restrict__ val,
-    const uint32_t *__restrict__ pbase, int shift, int64_t nrows, int64_t chunk,
+    const uint32_t *__restrict__ pbase, int shift, const Sched sc,
     const VT *__restrict__ x, void *__restrict__ yv, const VT *__restrict__ b,
     const VT *__restrict__ o1, const VT *__restrict__ isd, const VT *__restrict__ rr,
     VT *__restrict__ out2, VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last,
@@ -190,8 +280,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     if (done != nullptr && *done)
         return;
     const int64_t lb = logical_block();
-    const int64_t start = lb * chunk;
-    const int64_t end = start + chunk < nrows ? start + chunk : nrows;
     const int tr = threadIdx.x / L, k0 = (threadIdx.x % L) * W;
     double part[ND > 0 ? ND : 1][W];
 #pragma unroll
@@ -200,20 +288,44 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         for (int w = 0; w < W; ++w)
             part[d][w] = 0.0;
 
-    // row pointers of the next row are loaded one iteration ahead
+    for (int kt = 0; kt < sc.tpw; ++kt) {
+    const int64_t k = lb * sc.tpw + kt;
+    if (k >= sc.ntiles)
+        break;
+    int64_t start, end;
+    tile_rows(sc, k, start, end);
+    // Pipeline: the row pointers and panel base of the row two passes ahead and the
+    // first chunk of the next row are loaded while this row's x loads are in flight.
+    constexpr bool kPrefetch = OP != OP_OFFBLOCK;
     int64_t i = start + tr;
-    uint32_t jb = 0, je = 0;
+    uint32_t jb = 0, je = 0, bs = 0, nb = 0, ne = 0, nbs = 0;
     if (i < end) {
         jb = row[i];
         je = row[i + 1];
+        bs = panel_base<IT>(pbase, shift, i);
     }
+    if (i + RPB < end) {
+        nb = row[i + RPB];
+        ne = row[i + RPB + 1];
+        nbs = panel_base<IT>(pbase, shift, i + RPB);
+    }
+    Chunk<VT, NV, W> ch;
+    if constexpr (kPrefetch)
+        load_chunk<VT, IT, NV, W>(jb, je, col, val, ch);
+    // Drain the prologue loads once per tile: the row loop's head then merges only its
+    // own back edge, where the last y store may stay in flight.
+    __builtin_amdgcn_s_waitcnt(0);
     for (; i < end; i += RPB) {
         const size_t o = (size_t)i * NV + k0;
-        const uint32_t base = panel_base<IT>(pbase, shift, i);
-        uint32_t nb = 0, ne = 0;
-        if (i + RPB < end) {
-            nb = row[i + RPB];
-            ne = row[i + RPB + 1];
+        const uint32_t base = bs;
+        Chunk<VT, NV, W> nch;
+        if constexpr (kPrefetch)
+            load_chunk<VT, IT, NV, W>(nb, ne, col, val, nch);
+        uint32_t nnb = 0, nne = 0, nnbs = 0;
+        if (i + 2 * RPB < end) {
+            nnb = row[i + 2 * RPB];
+            nne = row[i + 2 * RPB + 1];
+            nnbs = panel_base<IT>(pbase, shift, i + 2 * RPB);
         }
         VT acc[W];
         if constexpr (OP == OP_ADD) {
@@ -239,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
                 }
             }
         } else {
-            row_acc<VT, IT, NV, W>(jb, je, col, val, base, x, k0, acc);
+            row_acc<VT, IT, NV, W>(jb, je, ch, col, val, base, x, k0, acc);
         }
         if constexpr (OP == OP_SET || OP == OP_ADD) {
             P yo;
@@ -354,24 +466,62 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         }
         jb = nb;
         je = ne;
+        bs = nbs;
+        nb = nnb;
+        ne = nne;
+        nbs = nnbs;
+        if constexpr (kPrefetch)
+            ch = nch;
+    }
     }
     if constexpr (ND > 0)
         block_partials<NV, W, ND>(part, partials, lb);
 }
 
-struct Launch {
-    int grid;
-    int64_t chunk;
-};
+} // namespace
 
-Launch plan(int64_t nrows, int rpb, int cap) {
-    int64_t need = (nrows + rpb - 1) / rpb;
-    int64_t g = std::min<int64_t>(need, cap);
-    g = std::max<int64_t>(8, (g + 7) / 8 * 8);
-    int64_t chunk = (nrows + g - 1) / g;
-    chunk = (chunk + rpb - 1) / rpb * rpb;
-    return {(int)g, chunk};
+// Tile schedule and grid of one launch: `rpb` rows per workgroup pass, at most `cap`
+// workgroups, `row_bytes` bytes of x and y per row (nv * sizeof(value)).
+Launch spmv_plan(int64_t n, int64_t nnz, int64_t F, int rpb, int cap, int64_t row_bytes) {
+    Launch L;
+    Sched &s = L.sc;
+    s.nrows = n;
+    if (g_tune.fold_enable && F > 0 && F < n && 2 * F * row_bytes > g_tune.fold_min_bytes) {
+        // strips: multiple of 8 (one XCD each, or a few), narrow enough that the rows
+        // touched between the first and last use of an x row (two strip rows of x, y
+        // and matrix) fit the budget
+        const int64_t per_row = 2 * row_bytes + 12 * std::max<int64_t>(1, nnz / std::max<int64_t>(1, n));
+        int64_t ns = 8;
+        while (2 * ((F + ns - 1) / ns) * per_row > g_tune.strip_budget && (F + ns - 1) / ns > 2 * rpb)
+            ns *= 2;
+        s.fold = F;
+        s.strip = (F + ns - 1) / ns;
+        s.na = (n + F - 1) / F;
+        int64_t t = std::min<int64_t>((int64_t)g_tune.tile_passes * rpb, s.strip);
+        t = std::max<int64_t>(rpb, t / rpb * rpb);
+        s.tile = (int32_t)t;
+        s.tps = (int32_t)((s.strip + t - 1) / t);
+        s.ntiles = ((F + s.strip - 1) / s.strip) * s.na * s.tps;
+    } else {
+        int64_t need = (n + rpb - 1) / rpb;
+        int64_t g = std::min<int64_t>(need, cap);
+        g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+        int64_t chunk = (n + g - 1) / g;
+        chunk = (chunk + rpb - 1) / rpb * rpb;
+        s.fold = n;
+        s.strip = n;
+        s.na = 1;
+        s.tile = (int32_t)chunk;
+        s.tps = (int32_t)((n + chunk - 1) / chunk);
+        s.ntiles = s.tps;
+    }
+    s.tpw = (int32_t)((s.ntiles + cap - 1) / cap);
+    int64_t g = (s.ntiles + s.tpw - 1) / s.tpw;
+    L.grid = (int)std::max<int64_t>(8, (g + 7) / 8 * 8);
+    return L;
 }
+
+namespace {
 
 template <typename VT, typename IT, int NV, int OP>
 int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_t s) {
@@ -379,10 +529,10 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
-    Launch L = plan(A.nrows, RPB, cap);
+    const Launch L = spmv_plan(A.nrows, A.nnz, A.fold, RPB, cap, (int64_t)NV * (int64_t)sizeof(VT));
     hipLaunchKernelGGL((k_spmv<VT, IT, NV, OP>), dim3(L.grid), dim3(kBlock), 0, s, A.row,
                        (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
-                       A.nrows, L.chunk, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,
+                       L.sc, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,
                        (const VT *)a.isd, (const VT *)a.r, (VT *)a.out2, (VT *)a.out3, a.xupd,
                        (VT)a.c, a.last, a.x_zero, a.out_f32, a.partials, a.done, a.blk_rows);
     return ND > 0 ? L.grid : 0;

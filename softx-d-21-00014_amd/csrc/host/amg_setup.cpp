@@ -6,8 +6,12 @@
 //   strength      -a_ij >= theta * max_k(-a_ik) (sign relative to the diagonal),
 //                 rows whose |row sum| > max_row_sum * |a_ii| have no strong links
 //   coarsening    Ruge-Stueben first pass (measure = number of points that strongly
-//                 depend on a point), optional second pass
-//   interpolation modified classical interpolation (hypre interp type 0)
+//                 depend on a point), optional second pass; on the first
+//                 mg_agg_num_levels levels aggressive coarsening: a second Ruge-Stueben
+//                 pass over the C points with the long-range strength graph (at least
+//                 mg_num_paths paths of length <= 2)
+//   interpolation modified classical interpolation (hypre interp type 0); multipass
+//                 interpolation (hypre agg_interp_type 4) on aggressive levels
 //   truncation    drop |w| < trunc_factor * max|w|, keep at most Pmax entries,
 //                 rescale each row to its original sum
 //   Galerkin      A_c = R A P with R = P^T (hypre_base.inl:400-401)
@@ -157,7 +161,8 @@ std::vector<uint8_t> strength(const CsrHost &A, float theta, float max_row_sum) 
 enum : int8_t { UNDECIDED = 0, CPT = 1, FPT = -1 };
 
 // Ruge-Stueben first pass with bucket lists keyed by the measure
-std::vector<int8_t> rs_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, int second_pass) {
+std::vector<int8_t> rs_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, int second_pass,
+                               bool isolated_as_c = false) {
     const int64_t n = A.nrows;
     // S^T adjacency: who strongly depends on i
     std::vector<uint32_t> tr(n + 1, 0);
@@ -184,7 +189,7 @@ std::vector<int8_t> rs_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, 
         for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
             has_dep |= S[j] != 0;
         if (!has_dep && lam[i] == 0)
-            cf[i] = FPT; // isolated point: nothing to interpolate from
+            cf[i] = isolated_as_c ? CPT : FPT; // isolated point
         maxlam = std::max(maxlam, lam[i]);
     }
     const int64_t nb = 2 * maxlam + n + 2; // measures can grow while F points are made
@@ -413,6 +418,221 @@ CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
     return P;
 }
 
+// Second stage of aggressive coarsening: among the first-stage C points, j is a
+// long-range strong dependency of i when there are at least num_paths paths of length
+// <= 2 from i to j in the strength graph (i -> j, or i -> k -> j).
+CsrHost second_strength(const CsrHost &A, const std::vector<uint8_t> &S,
+                        const std::vector<int8_t> &cf, int num_paths,
+                        std::vector<int64_t> &cidx) {
+    const int64_t n = A.nrows;
+    cidx.assign(n, -1);
+    std::vector<int64_t> cpts;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT) {
+            cidx[i] = (int64_t)cpts.size();
+            cpts.push_back(i);
+        }
+    const int64_t nc = (int64_t)cpts.size();
+    std::vector<std::vector<uint32_t>> rows(nc);
+#pragma omp parallel
+    {
+        std::vector<int> cnt(nc, 0);
+        std::vector<uint32_t> touched;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t ci = 0; ci < nc; ++ci) {
+            const int64_t i = cpts[ci];
+            touched.clear();
+            auto add = [&](int64_t j) {
+                const int64_t cj = cidx[j];
+                if (cj < 0 || cj == ci)
+                    return;
+                if (cnt[cj]++ == 0)
+                    touched.push_back((uint32_t)cj);
+            };
+            for (uint32_t q = A.row[i]; q < A.row[i + 1]; ++q) {
+                if (!S[q])
+                    continue;
+                const int64_t k = A.col[q];
+                add(k);
+                for (uint32_t r = A.row[k]; r < A.row[k + 1]; ++r)
+                    if (S[r])
+                        add(A.col[r]);
+            }
+            std::sort(touched.begin(), touched.end());
+            for (uint32_t cj : touched) {
+                if (cnt[cj] >= num_paths)
+                    rows[ci].push_back(cj);
+                cnt[cj] = 0;
+            }
+        }
+    }
+    CsrHost G;
+    G.nrows = G.ncols = nc;
+    G.row.assign(nc + 1, 0);
+    for (int64_t ci = 0; ci < nc; ++ci)
+        G.row[ci + 1] = G.row[ci] + (uint32_t)rows[ci].size();
+    G.col.resize(G.row[nc]);
+    for (int64_t ci = 0; ci < nc; ++ci)
+        std::copy(rows[ci].begin(), rows[ci].end(), G.col.begin() + G.row[ci]);
+    G.val.assign(G.col.size(), -1.0);
+    return G;
+}
+
+// truncation: drop |w| < trunc * max|w|, keep the pmax largest, each step rescaled to
+// the original row sum
+void truncate_row(std::vector<std::pair<uint32_t, double>> &row, float trunc, int pmax) {
+    double sum0 = 0.0, mx = 0.0;
+    for (auto &e : row) {
+        sum0 += e.second;
+        mx = std::max(mx, std::fabs(e.second));
+    }
+    if (trunc > 0.0f) {
+        std::vector<std::pair<uint32_t, double>> kept;
+        for (auto &e : row)
+            if (std::fabs(e.second) >= (double)trunc * mx)
+                kept.push_back(e);
+        double s1 = 0.0;
+        for (auto &e : kept)
+            s1 += e.second;
+        if (s1 != 0.0 && kept.size() < row.size())
+            for (auto &e : kept)
+                e.second *= sum0 / s1;
+        row.swap(kept);
+    }
+    if (pmax > 0 && (int)row.size() > pmax) {
+        double s0 = 0.0;
+        for (auto &e : row)
+            s0 += e.second;
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<uint32_t, double> &a,
+                            const std::pair<uint32_t, double> &b) {
+                             return std::fabs(a.second) > std::fabs(b.second);
+                         });
+        row.resize(pmax);
+        double s1 = 0.0;
+        for (auto &e : row)
+            s1 += e.second;
+        if (s1 != 0.0)
+            for (auto &e : row)
+                e.second *= s0 / s1;
+    }
+    std::sort(row.begin(), row.end());
+}
+
+// Multipass interpolation (Stueben): pass 1 interpolates F points with strong C
+// neighbours directly (w_ij = -alpha a_ij / a_ii, alpha = sum of all off-diagonal
+// entries / sum of the interpolatory ones); pass p interpolates the remaining F points
+// through their strong neighbours of earlier passes: P_i = sum_k (-alpha a_ik / a_ii) P_k.
+CsrHost multipass(const CsrHost &A, const std::vector<uint8_t> &S, const std::vector<int8_t> &cf,
+                  float trunc, int pmax, int64_t &nc) {
+    const int64_t n = A.nrows;
+    std::vector<int64_t> cidx(n, -1);
+    nc = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT)
+            cidx[i] = nc++;
+    std::vector<int> pass(n, -1);
+    std::vector<int64_t> frontier;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT)
+            pass[i] = 0;
+    // breadth-first pass numbers over strong dependencies
+    for (int p = 1;; ++p) {
+        frontier.clear();
+        for (int64_t i = 0; i < n; ++i) {
+            if (pass[i] >= 0)
+                continue;
+            for (uint32_t q = A.row[i]; q < A.row[i + 1]; ++q)
+                if (S[q] && pass[A.col[q]] >= 0 && pass[A.col[q]] < p) {
+                    frontier.push_back(i);
+                    break;
+                }
+        }
+        if (frontier.empty())
+            break;
+        for (int64_t i : frontier)
+            pass[i] = p;
+    }
+    int maxpass = 0;
+    for (int64_t i = 0; i < n; ++i)
+        maxpass = std::max(maxpass, pass[i]);
+    std::vector<std::vector<std::pair<uint32_t, double>>> P(n);
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT)
+            P[i] = {{(uint32_t)cidx[i], 1.0}};
+    std::vector<std::vector<int64_t>> members(maxpass + 1);
+    for (int64_t i = 0; i < n; ++i)
+        if (pass[i] > 0)
+            members[pass[i]].push_back(i);
+    for (int p = 1; p <= maxpass; ++p) {
+        const std::vector<int64_t> &mem = members[p];
+#pragma omp parallel
+        {
+            std::vector<double> acc(nc, 0.0);
+            std::vector<uint32_t> touched;
+#pragma omp for schedule(dynamic, 256)
+            for (size_t q = 0; q < mem.size(); ++q) {
+                const int64_t i = mem[q];
+                double aii = 0.0, sum_all = 0.0, sum_int = 0.0;
+                for (uint32_t r = A.row[i]; r < A.row[i + 1]; ++r) {
+                    const uint32_t k = A.col[r];
+                    if (k == (uint32_t)i) {
+                        aii += A.val[r];
+                        continue;
+                    }
+                    sum_all += A.val[r];
+                    if (S[r] && pass[k] >= 0 && pass[k] < p)
+                        sum_int += A.val[r];
+                }
+                if (aii == 0.0 || sum_int == 0.0)
+                    continue;
+                const double alpha = sum_all / sum_int;
+                touched.clear();
+                for (uint32_t r = A.row[i]; r < A.row[i + 1]; ++r) {
+                    const uint32_t k = A.col[r];
+                    if (k == (uint32_t)i || !S[r] || pass[k] < 0 || pass[k] >= p)
+                        continue;
+                    const double w = -alpha * A.val[r] / aii;
+                    for (auto &e : P[k]) {
+                        if (acc[e.first] == 0.0)
+                            touched.push_back(e.first);
+                        acc[e.first] += w * e.second;
+                    }
+                }
+                std::vector<std::pair<uint32_t, double>> row;
+                for (uint32_t c : touched) {
+                    if (acc[c] != 0.0)
+                        row.emplace_back(c, acc[c]);
+                    acc[c] = 0.0;
+                }
+                std::sort(row.begin(), row.end());
+                row.erase(std::unique(row.begin(), row.end(),
+                                      [](const std::pair<uint32_t, double> &a,
+                                         const std::pair<uint32_t, double> &b) {
+                                          return a.first == b.first;
+                                      }),
+                          row.end());
+                truncate_row(row, trunc, pmax);
+                P[i] = row;
+            }
+        }
+    }
+    CsrHost Pm;
+    Pm.nrows = n;
+    Pm.ncols = nc;
+    Pm.row.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        Pm.row[i + 1] = Pm.row[i] + (uint32_t)P[i].size();
+    Pm.col.resize(Pm.row[n]);
+    Pm.val.resize(Pm.row[n]);
+    for (int64_t i = 0; i < n; ++i)
+        for (size_t q = 0; q < P[i].size(); ++q) {
+            Pm.col[Pm.row[i] + q] = P[i][q].first;
+            Pm.val[Pm.row[i] + q] = P[i][q].second;
+        }
+    return Pm;
+}
+
 uint64_t splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -466,7 +686,22 @@ void build(const CsrHost &A0, const SetupParams &p, Hierarchy &H) {
         std::vector<uint8_t> S = strength(Af, p.strength, p.max_row_sum);
         std::vector<int8_t> cf = rs_coarsen(Af, S, p.second_pass);
         int64_t nc = 0;
-        CsrHost P = interpolation(Af, S, cf, p, nc);
+        CsrHost P;
+        const int lev = (int)H.A.size() - 1;
+        if (lev < p.agg_num_levels) {
+            // aggressive coarsening: a second Ruge-Stueben pass on the first-stage C
+            // points with the long-range strength graph, then multipass interpolation
+            std::vector<int64_t> cidx;
+            CsrHost G = second_strength(Af, S, cf, p.num_paths, cidx);
+            std::vector<uint8_t> S2(G.col.size(), 1);
+            std::vector<int8_t> cf2 = rs_coarsen(G, S2, 0, true);
+            for (int64_t i = 0; i < Af.nrows; ++i)
+                if (cf[i] == CPT && cf2[cidx[i]] != CPT)
+                    cf[i] = FPT;
+            P = multipass(Af, S, cf, p.agg_trunc_factor, p.agg_pmax, nc);
+        } else {
+            P = interpolation(Af, S, cf, p, nc);
+        }
         if (nc == 0 || nc >= Af.nrows)
             break;
         CsrHost R = transpose(P);

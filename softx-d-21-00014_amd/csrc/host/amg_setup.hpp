@@ -26,6 +26,10 @@ struct SetupParams {
     float trunc_factor = 0.25f; // mg_trunc_factor
     int pmax = 4;             // mg_Pmax_elements
     int second_pass = 0;      // RS second pass (F-F strong pairs share a C point)
+    int agg_num_levels = 10;  // mg_agg_num_levels: levels coarsened aggressively
+    int num_paths = 3;        // mg_num_paths: paths of length <= 2 for a long-range link
+    float agg_trunc_factor = 0.25f; // mg_agg_trunc_factor
+    int agg_pmax = 4;         // mg_agg_Pmax_elements
     int eig_iters = 10;       // power iterations for the Chebyshev bounds
     uint64_t eig_seed = 12345;
 };

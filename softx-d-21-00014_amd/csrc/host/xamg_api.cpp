@@ -61,6 +61,10 @@ void xamg_amg_default_params(xamg_amg_params *p) {
     p->trunc_factor = d.trunc_factor;
     p->pmax = d.pmax;
     p->second_pass = d.second_pass;
+    p->agg_num_levels = d.agg_num_levels;
+    p->num_paths = d.num_paths;
+    p->agg_trunc_factor = d.agg_trunc_factor;
+    p->agg_pmax = d.agg_pmax;
     p->eig_iters = d.eig_iters;
     p->eig_seed = d.eig_seed;
 }
@@ -86,6 +90,10 @@ int xamg_amg_setup(int64_t n, const uint32_t *row, const uint32_t *col, const do
         sp.trunc_factor = p->trunc_factor;
         sp.pmax = p->pmax;
         sp.second_pass = p->second_pass;
+        sp.agg_num_levels = p->agg_num_levels;
+        sp.num_paths = std::max(1, (int)p->num_paths);
+        sp.agg_trunc_factor = p->agg_trunc_factor;
+        sp.agg_pmax = p->agg_pmax;
         sp.eig_iters = p->eig_iters;
         sp.eig_seed = p->eig_seed;
     }
@@ -451,6 +459,10 @@ int build_xh_params(GlobalParams &g, xh_params &p, SetupParams &sp, bool &mg) {
         sp.max_levels = (int)geti(l, "mg_max_levels", 25);
         sp.trunc_factor = getf(l, "mg_trunc_factor", 0.25f);
         sp.pmax = (int)geti(l, "mg_Pmax_elements", 4);
+        sp.agg_num_levels = (int)geti(l, "mg_agg_num_levels", 10);
+        sp.num_paths = std::max(1, (int)geti(l, "mg_num_paths", 3));
+        sp.agg_trunc_factor = getf(l, "mg_agg_trunc_factor", 0.25f);
+        sp.agg_pmax = (int)geti(l, "mg_agg_Pmax_elements", 4);
     } else {
         p.graph = 1;
         p.hsgs_block_rows = 64;

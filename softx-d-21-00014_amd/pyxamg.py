@@ -41,7 +41,9 @@ class Params(C.Structure):
 class AmgParams(C.Structure):
     _fields_ = [("strength", C.c_float), ("max_row_sum", C.c_float), ("coarse_size", C.c_int32),
                 ("max_levels", C.c_int32), ("trunc_factor", C.c_float), ("pmax", C.c_int32),
-                ("second_pass", C.c_int32), ("eig_iters", C.c_int32), ("eig_seed", C.c_uint64)]
+                ("second_pass", C.c_int32), ("agg_num_levels", C.c_int32),
+                ("num_paths", C.c_int32), ("agg_trunc_factor", C.c_float),
+                ("agg_pmax", C.c_int32), ("eig_iters", C.c_int32), ("eig_seed", C.c_uint64)]
 
 
 def role(method, max_iters=1, abs_tol=0.0, rel_tol=0.0, conv_check=0, relax_factor=0.99,
@@ -87,6 +89,11 @@ def _load():
         "xh_spmv": [vp, C.c_int, vp, vp, C.c_int],
         "xh_spmv_timed": [vp, C.c_int, vp, vp, C.c_int, C.c_int, P(f64)],
         "xh_dense_matvec": [i64, C.c_int, C.c_int, vp, vp, vp],
+        "xh_matrix_fold": [vp, i64, P(i64)],
+        "xh_csr_fold": [i64, i64, vp, vp, P(i64)],
+        "xh_spmv_schedule": [i64, i64, i64, C.c_int, C.c_int, C.c_int, P(i64), P(i64), P(i64),
+                             vp],
+        "xh_set_option": [C.c_char_p, i64],
         "xh_hier_create": [C.c_int, P(vp)],
         "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
                               vp, vp, f64, f64],
@@ -161,6 +168,31 @@ def cube(nx, ny=None, nz=None):
     val = np.zeros(nnz, np.float64)
     check(h.xamg_cube(nx, ny, nz, _ptr(row), _ptr(col), _ptr(val)), "xamg_cube")
     return row, col, val
+
+
+def csr_fold(row, col, ncols=None):
+    """The fold (dominant long-range column offset) xh_matrix_create would detect."""
+    n = len(row) - 1
+    f = C.c_int64()
+    check(hip().xh_csr_fold(n, n if ncols is None else ncols, _ptr(row), _ptr(col),
+                            C.byref(f)), "xh_csr_fold")
+    return f.value
+
+
+def spmv_schedule(nrows, nnz, fold, nv, prec=0, dots=False):
+    """(grid, tiles per workgroup, [ntiles, 2] row ranges in processing order) of an SpMV
+    launch."""
+    g, nt, tpw = C.c_int64(), C.c_int64(), C.c_int64()
+    check(hip().xh_spmv_schedule(nrows, nnz, fold, prec, nv, int(dots), C.byref(g),
+                                 C.byref(nt), C.byref(tpw), None), "xh_spmv_schedule")
+    rows = np.zeros((nt.value, 2), np.int64)
+    check(hip().xh_spmv_schedule(nrows, nnz, fold, prec, nv, int(dots), None, None, None,
+                                 rows.ctypes.data), "xh_spmv_schedule")
+    return g.value, tpw.value, rows
+
+
+def set_option(key, value):
+    check(hip().xh_set_option(key.encode(), int(value)), "xh_set_option")
 
 
 def rhs_uniform(seed, n, nv):
@@ -366,6 +398,12 @@ class Matrix:
 
     def spmv(self, nv, dx, dy, accumulate=False):
         check(hip().xh_spmv(self.m, nv, dx.ptr, dy.ptr, int(accumulate)), "xh_spmv")
+
+    def fold(self, set=None):
+        f = C.c_int64()
+        check(hip().xh_matrix_fold(self.m, -1 if set is None else int(set), C.byref(f)),
+              "xh_matrix_fold")
+        return f.value
 
     def spmv_timed(self, nv, dx, dy, reps, accumulate=False):
         ms = C.c_double()
