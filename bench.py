@@ -4,8 +4,11 @@
 Workload (the configuration the headline metric is quoted on): the reference's 7-point
 cube Laplacian (examples/common/generator/cube.inl) on a 128^3 grid, 32 right-hand sides
 uniform in [-1,1] (SplitMix64, seed 1), x0 = 0, MergedPBiCGStab preconditioned by one
-V-cycle of the classical AMG hierarchy with degree-4 Chebyshev smoothing (1 pre + 1 post,
-Direct coarse solve), FP64 throughout, relative residual 1e-8 per RHS.
+V-cycle of the AMG hierarchy with degree-4 Chebyshev smoothing (1 pre + 1 post,
+Direct coarse solve), relative residual 1e-8 per RHS.  The hierarchy is the host AMG setup
+with the MultiGrid keys of the reference's test configurations (aggressive coarsening on 2
+levels with 2 paths, coarse size 500).  `value` is the FP64 solve; the same workload with
+the FP32 hierarchy under FP64 Krylov is reported beside it ("mixed_precision").
 
 One step = one complete solve to tolerance with the matrices and b resident in HBM
 (setup excluded, as examples/test/tst_solver.h:102-108).  value = ms per RHS.
@@ -38,6 +41,7 @@ def main():
     ap.add_argument("--mg-prec", type=int, default=0)
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-sample-nv", type=int, default=4)
     args = ap.parse_args()
 
@@ -85,9 +89,49 @@ def main():
     # true relative residual of the returned solution, with the GPU SpMM
     A = X.Matrix(row, col, val, prec=0, index_mode=0)
     dax = X.DeviceArray(n * nv)
-    A.spmv(nv, dx, dax)
-    r = (b - dax.to_host()).reshape(-1, nv)
-    true_rel = float(np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b.reshape(-1, nv), axis=0)))
+    bn = np.linalg.norm(b.reshape(-1, nv), axis=0)
+
+    def true_rel_res(d):
+        A.spmv(nv, d, dax)
+        r = (b - dax.to_host()).reshape(-1, nv)
+        return float(np.max(np.linalg.norm(r, axis=0) / bn))
+
+    true_rel = true_rel_res(dx)
+    x64 = dx.to_host().reshape(-1, nv)
+
+    # second run of the same workload: FP32 hierarchy, smoothers and SpMM inside the
+    # preconditioner, FP64 Krylov vectors, dots and coarsest solve
+    mixed = None
+    if not args.no_mixed:
+        pm = X.default_params(method="MergedPBiCGStab", precond="MultiGrid",
+                              smoother="Chebyshev", poly_order=args.poly_order, sweeps=1,
+                              rel_tol=1e-8, abs_tol=0.0, max_iters=100, mg_prec=2,
+                              coarse_fp64=1, graph=1)
+        Sm = X.Solver(H, pm, nv)
+        dxm = X.DeviceArray(n * nv)
+        for _ in range(args.warmup):
+            dxm.zero()
+            Sm.solve_device(db, dxm)
+        X.hip().xh_synchronize()
+        m0 = time.perf_counter()
+        for _ in range(args.steps):
+            dxm.zero()
+            Sm.solve_device(db, dxm)
+        X.hip().xh_synchronize()
+        m1 = time.perf_counter()
+        itm, convm, _, relm = Sm.stats()
+        xm = dxm.to_host().reshape(-1, nv)
+        err = np.abs(xm - x64).max(0) / np.abs(x64).max(0)
+        mixed = {
+            "ms_per_rhs": round((m1 - m0) * 1e3 / args.steps / nv, 4),
+            "ms_per_step": round((m1 - m0) * 1e3 / args.steps, 3),
+            "iters": itm,
+            "converged": int(convm.sum()),
+            "true_rel_res": true_rel_res(dxm),
+            "max_rel_diff_vs_fp64_solution": float(err.max()),
+            "precision": "fp32 hierarchy/smoothers/SpMM in the V-cycle, fp64 Krylov, dots "
+                         "and coarsest solve",
+        }
 
     # SpMM roofline: finest operator, y = A x, nv RHS, HIP events on the launch stream
     info = A.info()
@@ -130,6 +174,7 @@ def main():
         "max_rel_res": float(rel_res.max()),
         "true_rel_res": true_rel,
         "setup_s": round(setup_s, 2),
+        "mixed_precision": mixed,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved / 1e9, 1),

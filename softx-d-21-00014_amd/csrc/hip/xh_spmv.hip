@@ -62,32 +62,27 @@ __device__ __forceinline__ uint32_t panel_base(const uint32_t *__restrict__ pb, 
         return pb[i >> shift];
 }
 
-// The nonzeros of a row go in chunks.  With L >= 2 lanes per row, the L lanes load a
-// chunk's column indices and values cooperatively (E entries per lane, one coalesced load
-// each) and broadcast them across the group with cross-lane shuffles; with one lane per
-// row the lane loads 8 entries itself.  The first chunk of the next row is loaded one
-// pass ahead (together with its row pointers and panel base), so a row of up to CH
-// nonzeros costs one exposed memory round trip: its x loads.  Only existing entries are
-// added (no +0 terms), so each RHS gets the reference's sequential sum exactly.
 // Broadcast of lane S of each L-lane group (L consecutive lanes, L | 64) to the whole
 // group, S a compile-time constant.  DPP (a VALU operand modifier, no LDS traffic):
 // quad_perm for L = 2, 4; row_newbcast (gfx90a+: every lane of a 16-lane row reads
 // lane S of that row) for L = 16, and two of them merged for L = 8.  L = 32 goes
-// through ds_bpermute.
+// through ds_bpermute.  The lanes of a group share one row, so a group is active or
+// inactive as a whole and every source lane read by an active lane is active: the DPP
+// moves need no "old" operand.
 template <int L, int S>
 __device__ __forceinline__ int bcast_i32(int v) {
     static_assert(S >= 0 && S < L, "source lane outside the group");
     if constexpr (L == 2) {
         constexpr int q = S | (S << 2) | ((2 + S) << 4) | ((2 + S) << 6);
-        return __builtin_amdgcn_update_dpp(0, v, q, 0xF, 0xF, false);
+        return __builtin_amdgcn_mov_dpp(v, q, 0xF, 0xF, false);
     } else if constexpr (L == 4) {
         constexpr int q = S | (S << 2) | (S << 4) | (S << 6);
-        return __builtin_amdgcn_update_dpp(0, v, q, 0xF, 0xF, false);
+        return __builtin_amdgcn_mov_dpp(v, q, 0xF, 0xF, false);
     } else if constexpr (L == 16) {
-        return __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xF, 0xF, false);
+        return __builtin_amdgcn_mov_dpp(v, 0x150 + S, 0xF, 0xF, false);
     } else if constexpr (L == 8) {
-        const int lo = __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, v, 0x158 + S, 0xF, 0xF, false);
+        const int lo = __builtin_amdgcn_mov_dpp(v, 0x150 + S, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp(v, 0x158 + S, 0xF, 0xF, false);
         return (threadIdx.x & 8) ? hi : lo;
     } else {
         const int lane = threadIdx.x & 63;
@@ -117,12 +112,21 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
+// The nonzeros of a row go in chunks.  With L >= 2 lanes per row, the first LL =
+// min(L, 16) lanes of the row's group load a chunk's column indices and values
+// cooperatively (E entries per lane, one coalesced load each) and broadcast them across
+// the group (bcast); with one lane per row the lane loads 8 entries itself.  The first
+// chunk of the next row is loaded one pass ahead (together with its row pointers and
+// panel base), so a row of up to CH nonzeros costs one exposed memory round trip: its x
+// loads.  Only existing entries are added (no +0 terms), so each RHS gets the
+// reference's sequential sum exactly.
 template <typename VT, int NV, int W>
 struct Chunk {
     static constexpr int BW = 8;
-    static constexpr int L = NV / W;                        // lanes sharing the row
-    static constexpr int E = L >= 2 ? (BW + L - 1) / L : BW; // entries per lane
-    static constexpr int CH = L >= 2 ? L * E : BW;           // entries per chunk (>= 8)
+    static constexpr int L = NV / W;                    // lanes sharing the row
+    static constexpr int LL = L < 16 ? L : 16;          // of which load the chunk
+    static constexpr int E = L >= 2 ? (BW + LL - 1) / LL : BW; // entries per loading lane
+    static constexpr int CH = L >= 2 ? LL * E : BW;     // entries per chunk: 8 or 16
     uint32_t c[E]; // raw column index (panel-relative for u16)
     VT v[E];
 };
@@ -130,21 +134,25 @@ struct Chunk {
 template <typename VT, typename IT, int NV, int W>
 __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
                                            const VT *__restrict__ val, Chunk<VT, NV, W> &ch) {
+    // Entries past the row end repeat the row's last entry (its x row exists), so the
+    // x loads of a batch need no select; their products are never added.
     using C = Chunk<VT, NV, W>;
     const int gl = C::L >= 2 ? (int)(threadIdx.x & 63) % C::L : 0;
+    const bool live = j < e && gl < C::LL;
 #pragma unroll
     for (int q = 0; q < C::E; ++q) {
-        const uint32_t jj = j + (C::L >= 2 ? (uint32_t)(q * C::L + gl) : (uint32_t)q);
-        const bool ok = jj < e;
-        ch.c[q] = ok ? (uint32_t)col[jj] : 0u;
-        ch.v[q] = ok ? val[jj] : VT(0);
+        const uint32_t jj = j + (C::L >= 2 ? (uint32_t)(q * C::LL + gl) : (uint32_t)q);
+        const uint32_t jc = jj < e ? jj : e - 1;
+        ch.c[q] = live ? (uint32_t)col[jc] : 0u;
+        ch.v[q] = live ? val[jc] : VT(0);
     }
 }
 
 // acc[w] + sum x[base + c][k0 + w] * v over the entries [j, min(j + CH, e)) of `ch`.
-// The x loads of a batch are unconditional (a slot past the row end reloads the batch's
-// first entry, which exists) and the products of such slots are dropped by a select, so
-// no load sits behind its own branch and the loads of a batch stay in flight together.
+// The x loads of a batch are unconditional (a slot past the row end reloads the row's
+// last entry), so no load sits behind a branch and the loads of a batch stay in flight
+// together; the accumulation of a slot past the row end is skipped (a branch that is
+// uniform over the row's lanes, and over the wave where all its rows are shorter).
 template <typename VT, int NV, int W>
 __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chunk<VT, NV, W> &ch,
                                               uint32_t base, const VT *__restrict__ x, int k0,
@@ -152,56 +160,48 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
     using C = Chunk<VT, NV, W>;
     constexpr int BW = C::BW, L = C::L;
     if constexpr (L >= 2) {
-#pragma unroll
-        for (int b0 = 0; b0 < C::CH; b0 += BW) {
-            // uniform within the row's lane group, so every broadcast source is active
-            if (j + b0 >= e)
-                break;
+        bool more = true; // uniform within the row's lane group
+        static_for<0, C::CH / BW>([&](auto bc) {
+            constexpr int b0 = decltype(bc)::value * BW;
+            // every broadcast source lane of the group is active: the row (and so `more`)
+            // is the same for all L lanes
+            more = more && (j + b0 < e);
+            if (!more)
+                return;
             uint32_t c[BW];
             static_for<0, BW>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                // entry b0 + q of the chunk sits in lane (b0 + q) % L, slot (b0 + q) / L
-                if (b0 == 0)
-                    c[q] = (uint32_t)bcast<L, q % L>((int)ch.c[q / L]);
-                else
-                    c[q] = (uint32_t)bcast<L, (q + BW) % L>((int)ch.c[(q + BW) / L]);
+                // entry t of the chunk sits in lane t % LL of the group, slot t / LL
+                constexpr int t = b0 + decltype(qc)::value;
+                c[t - b0] = (uint32_t)bcast<L, t % C::LL>((int)ch.c[t / C::LL]);
             });
             Pack<VT, W> xv[BW];
 #pragma unroll
-            for (int q = 0; q < BW; ++q) {
-                const uint32_t cq = (j + b0 + q < e) ? c[q] : c[0];
-                xv[q] = ld<VT, W>(x + (size_t)(base + cq) * NV + k0);
-            }
+            for (int q = 0; q < BW; ++q)
+                xv[q] = ld<VT, W>(x + (size_t)(base + c[q]) * NV + k0);
             static_for<0, BW>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const VT vq = b0 == 0 ? bcast<L, q % L>(ch.v[q / L])
-                                      : bcast<L, (q + BW) % L>(ch.v[(q + BW) / L]);
-                const bool ok = j + b0 + q < e;
+                constexpr int t = b0 + q;
+                if (j + t < e) {
+                    const VT vq = bcast<L, t % C::LL>(ch.v[t / C::LL]);
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const VT t = acc[w] + xv[q].v[w] * vq;
-                    acc[w] = ok ? t : acc[w];
+                    for (int w = 0; w < W; ++w)
+                        acc[w] = acc[w] + xv[q].v[w] * vq;
                 }
             });
-        }
+        });
     } else {
         if (j >= e)
             return;
         Pack<VT, W> xv[BW];
 #pragma unroll
-        for (int q = 0; q < BW; ++q) {
-            const uint32_t cq = (j + q < e) ? ch.c[q] : ch.c[0];
-            xv[q] = ld<VT, W>(x + (size_t)(base + cq) * NV + k0);
-        }
+        for (int q = 0; q < BW; ++q)
+            xv[q] = ld<VT, W>(x + (size_t)(base + ch.c[q]) * NV + k0);
 #pragma unroll
-        for (int q = 0; q < BW; ++q) {
-            const bool ok = j + q < e;
+        for (int q = 0; q < BW; ++q)
+            if (j + q < e)
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const VT t = acc[w] + xv[q].v[w] * ch.v[q];
-                acc[w] = ok ? t : acc[w];
-            }
-        }
+                for (int w = 0; w < W; ++w)
+                    acc[w] = acc[w] + xv[q].v[w] * ch.v[q];
     }
 }
 

@@ -29,8 +29,9 @@ def test_spmv_cube_bit_exact(gpu, O, nv, index_mode):
     assert np.array_equal(dy.to_host(), O.spmv(row, col, vals, x, nv, y=y0))
 
 
-@pytest.mark.parametrize("nv", [1, 4, 8, 32])
+@pytest.mark.parametrize("nv", NVS)
 def test_spmv_fp32_bit_exact(gpu, O, nv):
+    # rows up to 40 nonzeros: several chunks and batches for every lane-group width
     X = gpu
     rng = np.random.default_rng(11)
     row, col, val = random_csr(rng, 700, 700, 0.02, max_row=40)
@@ -41,7 +42,7 @@ def test_spmv_fp32_bit_exact(gpu, O, nv):
     assert np.array_equal(dy.to_host(), O.spmv(row, col, val, x, nv, f32=True))
 
 
-@pytest.mark.parametrize("nv", [1, 8, 32, 64])
+@pytest.mark.parametrize("nv", NVS)
 def test_spmv_irregular_rows_bit_exact(gpu, O, nv):
     # empty rows, long rows (> 16 nonzeros: several cooperative chunks), rectangular
     X = gpu
