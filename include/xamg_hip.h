@@ -116,6 +116,28 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
  * when 2*fold*nv*sizeof(value) exceeds it). */
 int xh_set_option(const char *key, int64_t value);
 
+/* Compressed block: the reference's compact-index CSR block, as csr_matrix::compress
+ * (src/primitives/matrix/detail/csr_matrix.inl:86-170) leaves it.  Empty rows are
+ * dropped (row i of the block is row row_ind[i] of y, row_ind ascending, NULL when no row
+ * was dropped) and empty columns are dropped (column c is column col_ind[c] + col_offset
+ * of the full operand, col_ind ascending by construction).  col holds nnz compact column
+ * ids of col_bytes = 1, 2 or 4 bytes each (the widths define_int_type picks,
+ * src/bit_operations.h:119-129); they are decoded in the SpMM kernel.  xh_spmv on such a
+ * matrix reads x as the compact vector of ncols rows (the reference's halo layout,
+ * csr_matvec.inl:121-140) and writes rows row_ind[i] of y (y has y_rows rows): y += A x
+ * (csr_Axpy) or y = A x on those rows only (csr_Ax_y, :227-249; other rows untouched).
+ * The other solver operations do not take compressed blocks. */
+int xh_matrix_create_compressed(int64_t y_rows, int64_t nrows, int64_t ncols, int64_t nnz,
+                                const uint32_t *row, const void *col, int col_bytes,
+                                const uint32_t *row_ind, const uint32_t *col_ind,
+                                int64_t col_offset, int64_t x_rows, const double *val, int prec,
+                                xh_matrix *out);
+/* x_c[c][k] = x_full[col_ind[c] + col_offset][k] for the compact columns of a compressed
+ * block (the decode col_ind[col[j]] + block_col_offset,
+ * src/primitives/matrix/csr_matrix.h:290-297, applied to the operand); x_full has x_rows
+ * rows, x_c has ncols rows, both nv-interleaved in the matrix precision */
+int xh_matrix_gather(xh_matrix A, int nv, const void *dx_full, void *dx_c);
+
 /* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
 int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);
 
@@ -141,7 +163,12 @@ typedef struct {
     int32_t coarse_fp64;     /* mg_prec 2: coarsest direct solve in fp64 */
     int32_t index_mode;      /* column index width of the finest operator, as above */
     int32_t graph;           /* 1: replay one captured iteration as a hipGraph */
-    int32_t reserved[4];
+    int32_t true_res_period; /* > 0: every k-th iteration the recurrence residual is
+                                replaced by b - A x (fp64) and rho = (r, r0) recomputed;
+                                the convergence test is unchanged.  No reference
+                                counterpart (0, the default, is the reference behaviour);
+                                not for MergedBiCGStab, which updates x after r */
+    int32_t reserved[3];
 } xh_params;
 
 /* Hierarchy (the output of hypre::get_full_hierarchy, hypre_hierarchy.inl:31-52):

@@ -30,7 +30,8 @@ class Params(C.Structure):
     _fields_ = [("solver", Role), ("precond", Role), ("pre", Role), ("post", Role),
                 ("coarse", Role), ("mg_cycle", C.c_int32), ("mg_prec", C.c_int32),
                 ("hsgs_ncores", C.c_int32), ("hsgs_block_rows", C.c_int32),
-                ("dot_ncores", C.c_int32), ("coarse_fp64", C.c_int32)]
+                ("dot_ncores", C.c_int32), ("coarse_fp64", C.c_int32),
+                ("true_res_period", C.c_int32)]
 
 
 class HierS(C.Structure):
@@ -118,6 +119,26 @@ def spmv(row, col, val, x, nv, y=None, ncols=None, f32=False):
     return y
 
 
+def compress(row, col, val, ncols):
+    """The reference's compress() of a CSR block: dict(row, col (compact ids, uint32),
+    row_ind, col_ind, nrows, ncols, width) with width the define_int_type code of ncols."""
+    c = csr(row, col, val, ncols)
+    n = len(row) - 1
+    r_out = np.zeros(n + 1, np.uint32)
+    c_out = np.zeros(len(col), np.uint32)
+    r_ind = np.zeros(max(n, 1), np.uint32)
+    c_ind = np.zeros(max(ncols, 1), np.uint32)
+    nr, nc = C.c_int64(), C.c_int64()
+    rc = lib().xo_compress(C.byref(c), ncols, _p(r_out), _p(c_out), _p(r_ind), _p(c_ind),
+                           C.byref(nr), C.byref(nc))
+    if rc:
+        raise RuntimeError(f"oracle compress failed with {rc}")
+    nr, nc = nr.value, nc.value
+    return dict(row=r_out[:nr + 1].copy(), col=c_out, row_ind=r_ind[:nr].copy(),
+                col_ind=c_ind[:nc].copy(), nrows=nr, ncols=nc,
+                width=lib().xo_define_int_type(nc))
+
+
 def cheby_coeffs(mn, mx, order, fraction):
     c = np.zeros(order)
     rc = lib().xo_cheby_coeffs(mn, mx, order, fraction, _p(c))
@@ -187,6 +208,7 @@ def params_from(xp, dot_ncores=1):
     p.hsgs_block_rows = xp.hsgs_block_rows
     p.dot_ncores = dot_ncores
     p.coarse_fp64 = xp.coarse_fp64
+    p.true_res_period = xp.true_res_period
     return p
 
 

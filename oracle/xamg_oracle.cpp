@@ -977,6 +977,7 @@ struct Params {
     int mg_prec = 0;
     int hsgs_ncores = 1, hsgs_block_rows = 0;
     int coarse_fp64 = 0;
+    int true_res_period = 0;
 };
 
 template <typename T>
@@ -1287,11 +1288,23 @@ Solver<float> *MultiGrid::get<float>(AnySolver &s) { return s.f.get(); }
 struct Krylov : Solver<double> {
     const Csr<double> *A = nullptr;
     Solver<double> *precond = nullptr;
+    int true_res_period = 0;
     void get_residual(const Vec<double> &x, const Vec<double> &b, Vec<double> &r,
                       Vec<double> &res) { /* solver.h:243-256 */
         Ax_y(*A, x, r);
         axpby_s(1.0, b, -1.0, r);
         dot(r, r, res);
+    }
+    // Periodic residual replacement (no reference counterpart): on every k-th iteration
+    // the recurrence residual r, just updated, is replaced by b - A x (the same
+    // arithmetic as get_residual) and rho_new = (r, r0) is taken from it.
+    void replace_residual(int it, const Vec<double> &x, const Vec<double> &b, Vec<double> &r,
+                          const Vec<double> &r0, Vec<double> &rho_new) {
+        if (true_res_period <= 0 || it % true_res_period != 0)
+            return;
+        Ax_y(*A, x, r);
+        axpby_s(1.0, b, -1.0, r);
+        dot(r, r0, rho_new);
     }
 };
 
@@ -1351,6 +1364,7 @@ struct PBiCGStab : Krylov { /* pbicgstab.inl:44-237 */
             axpby_z(a1, s, S.omega, t, r);
             scal(a_1, S.omega);
             dot(r, r0, S.scal1);
+            replace_residual((int)it, x, b, r, r0, S.scal1);
             xdivy_z(S.scal1, S.rho, S.scal2);
             xdivy_z(S.alpha, S.omega, S.scal3);
             ax_y(S.scal2, S.scal3, S.beta);
@@ -1404,6 +1418,7 @@ struct MergedPBiCGStab : Krylov { /* merged_pbicgstab.inl:44-225 */
             if (converged(S.res, S.rho0, iconv))
                 return;
             group4(s, t, r0, r, S.omega, S.scal1);
+            replace_residual((int)it, x, b, r, r0, S.scal1);
             xdivy_z(S.scal1, S.rho, S.scal2);
             xdivy_z(S.alpha, S.omega, S.scal3);
             ax_y(S.scal2, S.scal3, S.beta);
@@ -1459,6 +1474,7 @@ struct BiCGStab : Krylov { /* bicgstab.inl:44-221 */
             axpby_z(a1, s, S.omega, t, r);
             scal(a_1, S.omega);
             dot(r, r0, S.scal1);
+            replace_residual((int)it, x, b, r, r0, S.scal1);
             xdivy_z(S.scal1, S.rho, S.scal2);
             xdivy_z(S.alpha, S.omega, S.scal3);
             ax_y(S.scal2, S.scal3, S.beta);
@@ -1534,6 +1550,7 @@ static Params params_of(const xo_params *p) {
     P.hsgs_ncores = std::max(1, p->hsgs_ncores);
     P.hsgs_block_rows = p->hsgs_block_rows;
     P.coarse_fp64 = p->coarse_fp64;
+    P.true_res_period = p->true_res_period;
     return P;
 }
 
@@ -1603,6 +1620,10 @@ static int build_set(const xo_hier *H, const xo_params *prm, SolverSet &Z) {
     if (auto *k = dynamic_cast<Krylov *>(Z.S)) {
         k->A = &Z.A0;
         k->precond = precond;
+        // MergedBiCGStab updates x only after r (group5), so b - A x is not r's residual
+        if (P.true_res_period > 0 && P.solver.method == XO_MERGED_BICGSTAB)
+            return -5;
+        k->true_res_period = P.true_res_period;
     }
     return g_err;
 }

@@ -66,6 +66,8 @@ typedef struct {
     int32_t hsgs_block_rows; /* >0: per-level block count ceil(n/block_rows) */
     int32_t dot_ncores;      /* emulated core count for partial sums of dots (>=1) */
     int32_t coarse_fp64;     /* mg_prec==2: coarsest direct solve kept in fp64 */
+    int32_t true_res_period; /* >0: every k-th iteration r = b - A x, rho = (r, r0)
+                                (no reference counterpart; 0 = reference behaviour) */
 } xo_params;
 
 typedef struct {

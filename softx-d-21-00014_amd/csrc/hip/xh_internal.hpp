@@ -15,7 +15,8 @@ constexpr int kMaxNV = 64;
 constexpr int kMaxPartials = 1024; // workgroups that write dot partials
 
 enum Prec : int { F64 = 0, F32 = 1 };
-enum Idx : int { IDX_U32 = 0, IDX_U16 = 1 }; // u16: panel-relative column (col = base + u16)
+// u16: panel-relative column (col = base + u16); u8: compact column of a compressed block
+enum Idx : int { IDX_U32 = 0, IDX_U16 = 1, IDX_U8 = 2 };
 
 // A device CSR matrix (rows stored diagonal first, ascending columns after it).
 struct Mat {
@@ -30,7 +31,14 @@ struct Mat {
     void *inv_diag = nullptr;      // nrows, prec of the matrix
     void *inv_sqrt_diag = nullptr; // nrows, prec of the matrix
     int64_t fold = 0;          // dominant long-range column offset (0: none), see Sched
-    size_t bytes = 0;          // device bytes owned
+    // compressed block (the reference's compress(): empty rows and columns dropped):
+    // row i of the block is row row_ind[i] of y, column c is column col_ind[c] + col_offset
+    // of the full operand; x of the SpMV is the compact vector of ncols rows
+    uint32_t *row_ind = nullptr; // nrows, or null (row i is row i)
+    uint32_t *col_ind = nullptr; // ncols, or null
+    int64_t col_offset = 0;
+    int64_t y_rows = 0;          // rows of y (block rows before compression)
+    size_t bytes = 0;            // device bytes owned
 };
 
 // Row schedule of an SpMV launch.  The rows are viewed as a (na x fold) array (row
@@ -181,6 +189,9 @@ void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int
 
 // smoother / transfer helpers
 // `done` (nullable): device flag; when set the launch is a no-op
+// x_c[c][k] = x_full[col_ind[c] + off][k], c < nc (the compressed block's column decode)
+void gather_cols(int64_t nc, int nv, int prec, const uint32_t *col_ind, int64_t off,
+                 const void *x_full, void *x_c, hipStream_t s);
 void cheb_init_zero(int64_t n, int nv, int prec, const void *b, const void *isd, double c,
                     void *r, void *u_or_x, int last, const int *done, hipStream_t s);
 void jacobi_update(int64_t n, int nv, int prec, void *x, const void *b, const void *r,

@@ -213,13 +213,14 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
     }
     if (h.nrows == h.ncols)
         M.fold = csr_fold(h.nrows, h.ncols, h.row.data(), h.col.data(), 256);
+    M.y_rows = h.nrows;
     M.bytes = acc;
     return 0;
 }
 
 void free_mat(Mat &M) {
     for (void *p : {(void *)M.row, M.col, (void *)M.panel_base, M.val, M.inv_diag,
-                    M.inv_sqrt_diag})
+                    M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind})
         if (p)
             (void)hipFree(p);
     M = Mat();
@@ -374,9 +375,10 @@ struct xh_solver_s {
     int hist_len = 0;
     const int *dflag = nullptr; // `done` gate of the launches being enqueued
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    const double *g_x = nullptr; // x pointer the graph was captured with
+    hipGraphExec_t gexec = nullptr, gexec_rep = nullptr; // plain / residual-replacing
+    hipGraph_t graph = nullptr, graph_rep = nullptr;
+    const double *g_x = nullptr, *g_b = nullptr; // pointers the graphs were captured with
+    const double *bcur = nullptr;                // b of the solve being enqueued
     size_t bytes = 0;
     int err = 0;
 };
@@ -571,7 +573,21 @@ void enqueue_init(xh_solver_s *S, const double *b, double *x) {
     finalize(FIN_INIT, S->st, S->partials, np, S->nv, S->stream);
 }
 
-void enqueue_iteration(xh_solver_s *S, double *x) {
+// r = b - A x (fp64) and partials of (r, r0), replacing the recurrence residual and the
+// rho_new partials of the r update just enqueued; returns the partial count
+int enqueue_replacement(xh_solver_s *S, double *x) {
+    SpmvArgs a = args(S);
+    a.b = S->bcur;
+    spmv(S->A0, S->nv, x, S->r, OP_RESID, a, S->stream);
+    VecArgs vd;
+    vd.st = S->st;
+    vd.r = S->r;
+    vd.r0 = S->r0;
+    vd.partials = S->partials;
+    return vec_op(V_DOT1, S->n, S->nv, vd, S->stream);
+}
+
+void enqueue_iteration(xh_solver_s *S, double *x, bool replace) {
     const int m = S->prm.solver.method;
     const int nv = S->nv;
     hipStream_t st = S->stream;
@@ -637,6 +653,8 @@ void enqueue_iteration(xh_solver_s *S, double *x) {
     vx.r0 = S->r0;
     vx.partials = S->partials;
     np = vec_op(V_XR_UPDATE, S->n, nv, vx, st);
+    if (replace)
+        np = enqueue_replacement(S, x);
     finalize(FIN_BETA, S->st, S->partials, np, nv, st);
     VecArgs vp;
     vp.st = S->st;
@@ -644,6 +662,21 @@ void enqueue_iteration(xh_solver_s *S, double *x) {
     vp.r = S->r;
     vp.v = S->v;
     vec_op(V_P_UPDATE, S->n, nv, vp, st);
+}
+
+void free_graphs(xh_solver_s *S) {
+    for (hipGraphExec_t *e : {&S->gexec, &S->gexec_rep})
+        if (*e) {
+            (void)hipGraphExecDestroy(*e);
+            *e = nullptr;
+        }
+    for (hipGraph_t *g : {&S->graph, &S->graph_rep})
+        if (*g) {
+            (void)hipGraphDestroy(*g);
+            *g = nullptr;
+        }
+    S->g_x = nullptr;
+    S->g_b = nullptr;
 }
 
 int check_launch() {
@@ -674,25 +707,31 @@ int solve_impl(xh_solver_s *S, const double *db, double *dx) {
     if (rc)
         return rc;
     const bool use_graph = S->prm.graph != 0;
-    if (use_graph && (S->gexec == nullptr || S->g_x != dx)) {
-        if (S->gexec) {
-            (void)hipGraphExecDestroy(S->gexec);
-            (void)hipGraphDestroy(S->graph);
-            S->gexec = nullptr;
-            S->graph = nullptr;
-        }
+    const int period = S->prm.true_res_period;
+    S->bcur = db;
+    if (use_graph && (S->gexec == nullptr || S->g_x != dx || S->g_b != db)) {
+        free_graphs(S);
         XH_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_iteration(S, dx);
+        enqueue_iteration(S, dx, false);
         XH_HIP(hipStreamEndCapture(S->stream, &S->graph));
         XH_HIP(hipGraphInstantiate(&S->gexec, S->graph, nullptr, nullptr, 0));
+        if (period > 0) {
+            XH_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(S, dx, true);
+            XH_HIP(hipStreamEndCapture(S->stream, &S->graph_rep));
+            XH_HIP(hipGraphInstantiate(&S->gexec_rep, S->graph_rep, nullptr, nullptr, 0));
+        }
         S->g_x = dx;
+        S->g_b = db;
     }
     const int maxit = std::max(1, S->prm.solver.max_iters) + 1;
     for (int it = 0; it < maxit; ++it) {
+        // replay `it` runs iteration it + 1 (or nothing once the solve is done)
+        const bool replace = period > 0 && (it + 1) % period == 0;
         if (use_graph) {
-            XH_HIP(hipGraphLaunch(S->gexec, S->stream));
+            XH_HIP(hipGraphLaunch(replace ? S->gexec_rep : S->gexec, S->stream));
         } else {
-            enqueue_iteration(S, dx);
+            enqueue_iteration(S, dx, replace);
             if ((rc = check_launch()))
                 return rc;
         }
@@ -844,7 +883,7 @@ int xh_matrix_info(xh_matrix A, int64_t *nrows, int64_t *ncols, int64_t *nnz, in
     if (prec)
         *prec = A->m.prec;
     if (index_bits)
-        *index_bits = A->m.idx == IDX_U16 ? 16 : 32;
+        *index_bits = A->m.idx == IDX_U8 ? 8 : (A->m.idx == IDX_U16 ? 16 : 32);
     if (panel_rows)
         *panel_rows = A->m.idx == IDX_U16 ? ((int64_t)1 << A->m.panel_shift) : 0;
     if (device_bytes)
@@ -976,6 +1015,11 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
         return fail(XH_ENOTSUP, "xh_solver_create: solver method must be a BiCGStab variant");
     if (is_preconditioned(m) && prm->precond.method == XH_NONE)
         return fail(XH_EINVAL, "xh_solver_create: preconditioned method without preconditioner");
+    if (prm->true_res_period < 0)
+        return fail(XH_EINVAL, "xh_solver_create: true_res_period must be >= 0");
+    if (prm->true_res_period > 0 && m == XH_MERGED_BICGSTAB)
+        return fail(XH_ENOTSUP, "xh_solver_create: true_res_period is not available for "
+                                "MergedBiCGStab (x is updated after r)");
     const int nl = H->nlevels;
     for (int l = 0; l < nl; ++l)
         if (H->A[l].nrows == 0 || (l + 1 < nl && H->P[l].ncols != H->A[l + 1].nrows))
@@ -1212,10 +1256,7 @@ int xh_solver_destroy(xh_solver S) {
         return 0;
     if (S->stream)
         (void)hipStreamSynchronize(S->stream);
-    if (S->gexec)
-        (void)hipGraphExecDestroy(S->gexec);
-    if (S->graph)
-        (void)hipGraphDestroy(S->graph);
+    free_graphs(S);
     for (Level &L : S->lv) {
         if (L.own_A)
             free_mat(L.A);
@@ -1315,5 +1356,142 @@ extern "C" int xh_matrix_fold(xh_matrix A, int64_t set, int64_t *fold) {
     }
     if (fold)
         *fold = A->m.fold;
+    return 0;
+}
+
+// Compressed block upload (csr_matrix::compress, csr_matrix.inl:86-170): indices are kept
+// at the width given (1, 2 or 4 bytes) and decoded in the SpMM kernel.
+extern "C" int xh_matrix_create_compressed(int64_t y_rows, int64_t nrows, int64_t ncols,
+                                           int64_t nnz, const uint32_t *row, const void *col,
+                                           int col_bytes, const uint32_t *row_ind,
+                                           const uint32_t *col_ind, int64_t col_offset,
+                                           int64_t x_rows, const double *val, int prec,
+                                           xh_matrix *out) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (!out || !row || (nnz > 0 && (!col || !val)) || nrows < 0 || ncols < 0 || y_rows < 0 ||
+        col_offset < 0 || x_rows < 0)
+        return fail(XH_EINVAL, "xh_matrix_create_compressed: bad arguments");
+    if (col_bytes != 1 && col_bytes != 2 && col_bytes != 4)
+        return fail(XH_EINVAL, "xh_matrix_create_compressed: col_bytes must be 1, 2 or 4");
+    if (prec != F64 && prec != F32)
+        return fail(XH_EINVAL, "xh_matrix_create_compressed: prec must be 0 (f64) or 1 (f32)");
+    if ((int64_t)row[nrows] != nnz || row[0] != 0)
+        return fail(XH_EINVAL, "xh_matrix_create_compressed: row pointer does not match nnz");
+    for (int64_t i = 0; i < nrows; ++i)
+        if (row[i + 1] < row[i])
+            return fail(XH_EINVAL, "xh_matrix_create_compressed: row pointer not monotone");
+    if (row_ind == nullptr && nrows != y_rows)
+        return fail(XH_EINVAL, "xh_matrix_create_compressed: row_ind needed when rows were dropped");
+    if (row_ind != nullptr)
+        for (int64_t i = 0; i < nrows; ++i)
+            if ((int64_t)row_ind[i] >= y_rows || (i > 0 && row_ind[i] <= row_ind[i - 1]))
+                return fail(XH_EINVAL,
+                            "xh_matrix_create_compressed: row_ind must ascend below y_rows");
+    if (col_ind != nullptr)
+        for (int64_t c = 0; c < ncols; ++c)
+            if ((int64_t)col_ind[c] + col_offset >= x_rows)
+                return fail(XH_EINVAL,
+                            "xh_matrix_create_compressed: col_ind + col_offset beyond x_rows");
+    std::vector<uint32_t> cv(nnz);
+    for (int64_t j = 0; j < nnz; ++j) {
+        cv[j] = col_bytes == 1   ? ((const uint8_t *)col)[j]
+                : col_bytes == 2 ? ((const uint16_t *)col)[j]
+                                 : ((const uint32_t *)col)[j];
+        if ((int64_t)cv[j] >= ncols)
+            return fail(XH_EINVAL, "xh_matrix_create_compressed: column index out of range");
+    }
+    auto *m = new xh_matrix_s();
+    Mat &M = m->m;
+    M.nrows = nrows;
+    M.ncols = ncols;
+    M.nnz = nnz;
+    M.prec = prec;
+    M.y_rows = y_rows;
+    M.col_offset = col_offset;
+    M.idx = col_bytes == 1 ? IDX_U8 : (col_bytes == 2 ? IDX_U16 : IDX_U32);
+    M.panel_shift = 32; // one panel with base 0 for every row (rows < 2^32)
+    size_t acc = 0;
+    auto fail_free = [&](int code) {
+        free_mat(M);
+        delete m;
+        return code;
+    };
+    if ((rc = dmalloc(&M.row, nrows + 1, acc)))
+        return fail_free(rc);
+    hipError_t e = hipMemcpy(M.row, row, (nrows + 1) * 4, hipMemcpyHostToDevice);
+    if (M.idx != IDX_U32 && e == hipSuccess) {
+        if ((rc = dmalloc(&M.panel_base, 1, acc)))
+            return fail_free(rc);
+        const uint32_t zero = 0;
+        e = hipMemcpy(M.panel_base, &zero, 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && nnz > 0) {
+        if (M.idx == IDX_U8) {
+            uint8_t *d = nullptr;
+            if ((rc = dmalloc(&d, nnz, acc)))
+                return fail_free(rc);
+            M.col = d;
+            std::vector<uint8_t> c8(cv.begin(), cv.end());
+            e = hipMemcpy(d, c8.data(), nnz, hipMemcpyHostToDevice);
+        } else if (M.idx == IDX_U16) {
+            uint16_t *d = nullptr;
+            if ((rc = dmalloc(&d, nnz, acc)))
+                return fail_free(rc);
+            M.col = d;
+            std::vector<uint16_t> c16(cv.begin(), cv.end());
+            e = hipMemcpy(d, c16.data(), nnz * 2, hipMemcpyHostToDevice);
+        } else {
+            uint32_t *d = nullptr;
+            if ((rc = dmalloc(&d, nnz, acc)))
+                return fail_free(rc);
+            M.col = d;
+            e = hipMemcpy(d, cv.data(), nnz * 4, hipMemcpyHostToDevice);
+        }
+    }
+    if (e == hipSuccess && nnz > 0) {
+        if (prec == F64) {
+            double *d = nullptr;
+            if ((rc = dmalloc(&d, nnz, acc)))
+                return fail_free(rc);
+            M.val = d;
+            e = hipMemcpy(d, val, nnz * 8, hipMemcpyHostToDevice);
+        } else {
+            std::vector<float> vf(val, val + nnz);
+            float *d = nullptr;
+            if ((rc = dmalloc(&d, nnz, acc)))
+                return fail_free(rc);
+            M.val = d;
+            e = hipMemcpy(d, vf.data(), nnz * 4, hipMemcpyHostToDevice);
+        }
+    }
+    if (e == hipSuccess && row_ind != nullptr && nrows > 0) {
+        if ((rc = dmalloc(&M.row_ind, nrows, acc)))
+            return fail_free(rc);
+        e = hipMemcpy(M.row_ind, row_ind, nrows * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && col_ind != nullptr && ncols > 0) {
+        if ((rc = dmalloc(&M.col_ind, ncols, acc)))
+            return fail_free(rc);
+        e = hipMemcpy(M.col_ind, col_ind, ncols * 4, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess)
+        return fail_free(fail(XH_EHIP, std::string("xh_matrix_create_compressed: ") +
+                                           hipGetErrorString(e)));
+    M.bytes = acc;
+    *out = m;
+    return 0;
+}
+
+extern "C" int xh_matrix_gather(xh_matrix A, int nv, const void *dx_full, void *dx_c) {
+    if (!A || !nv_ok(nv) || !dx_full || !dx_c)
+        return fail(XH_EINVAL, "xh_matrix_gather: bad arguments");
+    if (A->m.col_ind == nullptr)
+        return fail(XH_EINVAL, "xh_matrix_gather: the matrix has no col_ind");
+    gather_cols(A->m.ncols, nv, A->m.prec, A->m.col_ind, A->m.col_offset, dx_full, dx_c,
+                g_stream);
+    XH_HIP(hipGetLastError());
+    XH_HIP(hipStreamSynchronize(g_stream));
     return 0;
 }

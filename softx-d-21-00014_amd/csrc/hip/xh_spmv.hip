@@ -7,7 +7,8 @@
 // is accumulated in stored nonzero order exactly as the reference
 // (acc = acc + x*val, no FMA contraction: the file is built with -ffp-contract=off).
 // Column indices are u32 or u16 panel-relative (col = base[row >> shift] + u16), decoded
-// in-kernel.  The epilogues fuse the level-1 work that follows an SpMV in the reference
+// in-kernel; a compressed block (the reference's compact-index format) has u8/u16/u32
+// compact columns and writes row row_ind[i] of y.  The epilogues fuse the level-1 work that follows an SpMV in the reference
 // solvers (residual, dot partials, Chebyshev steps, mixed-precision prolongation, HSGS
 // off-block residual).
 //
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const VT *__restrict__ o1, const VT *__restrict__ isd, const VT *__restrict__ rr,
     VT *__restrict__ out2, VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last,
     int x_zero, int out_f32, double *__restrict__ partials, const int *__restrict__ done,
-    int64_t blk_rows) {
+    int64_t blk_rows, const uint32_t *__restrict__ rind) {
     constexpr int ND = NDots<OP>::value;
     constexpr int W = Width<VT, NV>::value;
     constexpr int L = NV / W;        // lanes per row
@@ -327,9 +328,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             nne = row[i + 2 * RPB + 1];
             nnbs = panel_base<IT>(pbase, shift, i + 2 * RPB);
         }
+        // output row: row_ind[i] for a compressed block (SET / ADD only)
+        const size_t oy = (OP == OP_SET || OP == OP_ADD) && rind != nullptr
+                              ? (size_t)rind[i] * NV + k0
+                              : o;
         VT acc[W];
         if constexpr (OP == OP_ADD) {
-            const P y0 = ld<VT, W>((const VT *)yv + o);
+            const P y0 = ld<VT, W>((const VT *)yv + oy);
 #pragma unroll
             for (int w = 0; w < W; ++w)
                 acc[w] = y0.v[w];
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
 #pragma unroll
             for (int w = 0; w < W; ++w)
                 yo.v[w] = acc[w];
-            st<VT, W>((VT *)yv + o, yo);
+            st<VT, W>((VT *)yv + oy, yo);
         } else if constexpr (OP == OP_RESID || OP == OP_OFFBLOCK) {
             const P bv = ld<VT, W>(b + o);
             if (OP == OP_RESID && out_f32) {
@@ -534,12 +539,23 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
                        (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
                        L.sc, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,
                        (const VT *)a.isd, (const VT *)a.r, (VT *)a.out2, (VT *)a.out3, a.xupd,
-                       (VT)a.c, a.last, a.x_zero, a.out_f32, a.partials, a.done, a.blk_rows);
+                       (VT)a.c, a.last, a.x_zero, a.out_f32, a.partials, a.done, a.blk_rows,
+                       A.row_ind);
     return ND > 0 ? L.grid : 0;
 }
 
 template <typename VT, typename IT, int NV>
 int by_op(const Mat &A, const void *x, void *y, SpmvOp op, const SpmvArgs &a, hipStream_t s) {
+    if (A.row_ind != nullptr && op != OP_SET && op != OP_ADD)
+        return -1; // a compressed block only takes y = A x and y += A x
+    if constexpr (sizeof(IT) == 1) {
+        // u8 columns only occur in compressed blocks
+        if (op == OP_SET)
+            return launch_t<VT, IT, NV, OP_SET>(A, x, y, a, s);
+        if (op == OP_ADD)
+            return launch_t<VT, IT, NV, OP_ADD>(A, x, y, a, s);
+        return -1;
+    }
     switch (op) {
     case OP_SET:
         return launch_t<VT, IT, NV, OP_SET>(A, x, y, a, s);
@@ -603,9 +619,14 @@ int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs
          hipStream_t s) {
     if (A.nrows == 0)
         return 0;
-    if (A.prec == F64)
+    if (A.prec == F64) {
+        if (A.idx == IDX_U8)
+            return by_nv<double, uint8_t>(A, nv, x, y, op, a, s);
         return A.idx == IDX_U16 ? by_nv<double, uint16_t>(A, nv, x, y, op, a, s)
                                 : by_nv<double, uint32_t>(A, nv, x, y, op, a, s);
+    }
+    if (A.idx == IDX_U8)
+        return by_nv<float, uint8_t>(A, nv, x, y, op, a, s);
     return A.idx == IDX_U16 ? by_nv<float, uint16_t>(A, nv, x, y, op, a, s)
                             : by_nv<float, uint32_t>(A, nv, x, y, op, a, s);
 }

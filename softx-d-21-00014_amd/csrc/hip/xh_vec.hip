@@ -467,6 +467,20 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t count, const T1 *__r
         b[e] = (T2)a[e];
 }
 
+// x_c[c][k] = x_full[col_ind[c] + off][k]: the column decode of a compressed block
+// (col_ind[col[j]] + block_col_offset, csr_matrix.h:290-297) applied to the operand
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather_cols(int64_t count, int nv,
+                                                        const uint32_t *__restrict__ col_ind,
+                                                        int64_t off, const T *__restrict__ xf,
+                                                        T *__restrict__ xc) {
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int64_t c = e / nv, k = e - c * nv;
+        xc[e] = xf[((int64_t)col_ind[c] + off) * nv + k];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill0(int64_t count, T *__restrict__ b,
                                                   const int *__restrict__ done) {
@@ -635,6 +649,19 @@ void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *do
     else
         hipLaunchKernelGGL((k_convert<float, float>), dim3(grid_for(count)), dim3(kBlock), 0, s,
                            count, (const float *)src, (float *)dst, done);
+}
+
+void gather_cols(int64_t nc, int nv, int prec, const uint32_t *col_ind, int64_t off,
+                 const void *x_full, void *x_c, hipStream_t s) {
+    const int64_t count = nc * nv;
+    if (count == 0)
+        return;
+    if (prec == F64)
+        hipLaunchKernelGGL(k_gather_cols<double>, dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, nv, col_ind, off, (const double *)x_full, (double *)x_c);
+    else
+        hipLaunchKernelGGL(k_gather_cols<float>, dim3(grid_for(count)), dim3(kBlock), 0, s,
+                           count, nv, col_ind, off, (const float *)x_full, (float *)x_c);
 }
 
 void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s) {

@@ -434,6 +434,8 @@ int build_xh_params(GlobalParams &g, xh_params &p, SetupParams &sp, bool &mg) {
     int rc;
     if ((rc = role_of(g.roles["solver"], p.solver)))
         return rc;
+    // extension of this build (no reference key): periodic residual replacement
+    p.true_res_period = (int32_t)geti(g.roles["solver"], "true_res_period", 0);
     p.precond.method = XH_NONE;
     p.pre.method = p.post.method = p.coarse.method = XH_NONE;
     mg = false;

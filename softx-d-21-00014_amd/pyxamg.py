@@ -35,7 +35,7 @@ class Params(C.Structure):
                 ("coarse", Role), ("mg_cycle", C.c_int32), ("mg_prec", C.c_int32),
                 ("hsgs_ncores", C.c_int32), ("hsgs_block_rows", C.c_int32),
                 ("coarse_fp64", C.c_int32), ("index_mode", C.c_int32), ("graph", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("true_res_period", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class AmgParams(C.Structure):
@@ -94,6 +94,9 @@ def _load():
         "xh_spmv_schedule": [i64, i64, i64, C.c_int, C.c_int, C.c_int, P(i64), P(i64), P(i64),
                              vp],
         "xh_set_option": [C.c_char_p, i64],
+        "xh_matrix_create_compressed": [i64, i64, i64, i64, vp, vp, C.c_int, vp, vp, i64, i64,
+                                        vp, C.c_int, P(vp)],
+        "xh_matrix_gather": [vp, C.c_int, vp, vp],
         "xh_hier_create": [C.c_int, P(vp)],
         "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
                               vp, vp, f64, f64],
@@ -387,6 +390,31 @@ class Matrix:
                                      _ptr(self.row), _ptr(self.col), _ptr(self.val), prec,
                                      index_mode, C.byref(self.m)), "xh_matrix_create")
 
+    @classmethod
+    def compressed(cls, y_rows, row, col, val, row_ind=None, col_ind=None, col_offset=0,
+                   x_rows=0, prec=0):
+        """A compressed block (include/xamg_hip.h: xh_matrix_create_compressed); col keeps
+        its dtype (uint8, uint16 or uint32) as the compact-index width."""
+        self = cls.__new__(cls)
+        self.row = np.ascontiguousarray(row, np.uint32)
+        self.col = np.ascontiguousarray(col)
+        assert self.col.dtype in (np.uint8, np.uint16, np.uint32)
+        self.val = np.ascontiguousarray(val, np.float64)
+        self.row_ind = None if row_ind is None else np.ascontiguousarray(row_ind, np.uint32)
+        self.col_ind = None if col_ind is None else np.ascontiguousarray(col_ind, np.uint32)
+        n = len(self.row) - 1
+        ncols = len(self.col_ind) if self.col_ind is not None else int(self.col.max()) + 1
+        self.m = C.c_void_p()
+        check(hip().xh_matrix_create_compressed(
+            y_rows, n, ncols, len(self.col), _ptr(self.row), _ptr(self.col),
+            self.col.dtype.itemsize, None if self.row_ind is None else _ptr(self.row_ind),
+            None if self.col_ind is None else _ptr(self.col_ind), col_offset, x_rows,
+            _ptr(self.val), prec, C.byref(self.m)), "xh_matrix_create_compressed")
+        return self
+
+    def gather(self, nv, dx_full, dx_c):
+        check(hip().xh_matrix_gather(self.m, nv, dx_full.ptr, dx_c.ptr), "xh_matrix_gather")
+
     def info(self):
         nr, nc, nnz, pr, bits, panel = (C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(),
                                         C.c_int(), C.c_int64())
@@ -432,7 +460,7 @@ def device_info():
 def default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Chebyshev",
                    poly_order=4, sweeps=1, rel_tol=1e-8, abs_tol=0.0, max_iters=50,
                    coarse="Direct", mg_prec=0, hsgs_block_rows=64, graph=1, index_mode=0,
-                   mg_cycle=0, relax=0.99, fraction=0.3, coarse_fp64=1):
+                   mg_cycle=0, relax=0.99, fraction=0.3, coarse_fp64=1, true_res_period=0):
     p = Params()
     p.solver = role(method, max_iters=max_iters, abs_tol=abs_tol, rel_tol=rel_tol,
                     conv_check=1)
@@ -449,4 +477,5 @@ def default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Cheb
     p.coarse_fp64 = coarse_fp64
     p.index_mode = index_mode
     p.graph = graph
+    p.true_res_period = true_res_period
     return p

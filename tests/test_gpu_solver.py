@@ -231,3 +231,32 @@ def test_reference_yaml_configs(gpu, O, name):
     assert np.isfinite(g["x"]).all() == np.isfinite(r["x"]).all()
     if np.isfinite(xr).all():
         assert rel_diff(g["x"], r["x"], nv) <= 1e-6
+
+
+@pytest.mark.parametrize("method", ["PBiCGStab", "MergedPBiCGStab", "BiCGStab"])
+@pytest.mark.parametrize("graph", [0, 1])
+def test_true_residual_replacement_parity(gpu, O, method, graph):
+    # every 2nd iteration r = b - A x (fp64) and rho = (r, r0), on the GPU (a second
+    # captured graph on those iterations) and in the oracle
+    # (the unpreconditioned case at the size and tolerance of the unpreconditioned parity
+    # test: over many unpreconditioned iterations the dot reduction order moves the count)
+    X = gpu
+    pre = "P" in method
+    N, nv = (20, 4) if pre else (12, 4)
+    levels = levels_for(X, N)
+    b = X.rhs_uniform(21, N ** 3, nv)
+    prm = X.default_params(method=method, smoother="Chebyshev", poly_order=3, graph=graph,
+                           true_res_period=2, rel_tol=1e-10 if pre else 1e-8, max_iters=400,
+                           precond="MultiGrid" if pre else None)
+    out, ref = run_both(X, O, levels, prm, nv, b)
+    assert out["conv"].all() and ref["conv"].all()
+    assert abs(out["iters"] - ref["iters"]) <= 1
+    assert rel_diff(out["x"], ref["x"], nv) <= 1e-6
+
+
+def test_true_residual_replacement_rejected_for_merged_bicgstab(gpu):
+    X = gpu
+    levels = levels_for(X, 8)
+    prm = X.default_params(method="MergedBiCGStab", precond=None, true_res_period=1)
+    with pytest.raises(X.XamgError):
+        X.Solver(X.Hier(levels), prm, 1)

@@ -307,3 +307,45 @@ def test_uneven_convergence_freezes_converged_rhs(O, X):
     # a converged RHS keeps the abs residual recorded at its first convergence
     for k in range(nv):
         assert r["abs_res"][k] == pytest.approx(h[first[k], k], rel=1e-12)
+
+
+@pytest.mark.parametrize("method", ["PBICGSTAB", "MERGED_PBICGSTAB", "BICGSTAB"])
+def test_true_residual_replacement(O, X, method):
+    # r <- b - A x every k-th iteration: in exact arithmetic the iterates do not change,
+    # so on a well-conditioned operator x and the iteration count stay put, and with a
+    # replacement every iteration the tested (recurrence) residual follows the true
+    # residual of the returned x closely
+    N, nv = 14, 3
+    row, col, val = O.cube(N)
+    levels = X.Amg(row, col, val).export()
+    H = O.Hier(levels)
+    b = X.rhs_uniform(8, N ** 3, nv)
+    A = sp.csr_matrix((val, col, row), shape=(N ** 3, N ** 3))
+    m = getattr(O, method)
+    res = {}
+    for k in (0, 1, 3):
+        p = mg_params(O, O.CHEBYSHEV, O.CHEBYSHEV, order=2)
+        if method == "BICGSTAB":
+            p.precond = O.role(O.NONE)
+        p.solver = O.role(m, max_iters=300, rel_tol=1e-10, conv_check=1)
+        p.true_res_period = k
+        res[k] = O.solve(H, p, nv, b)
+        assert res[k]["conv"].all()
+    x0 = res[0]["x"].reshape(-1, nv)
+    for k in (1, 3):
+        assert abs(res[k]["iters"] - res[0]["iters"]) <= 1
+        err = np.abs(res[k]["x"].reshape(-1, nv) - x0).max(0) / np.abs(x0).max(0)
+        assert err.max() < 1e-8
+    tr = np.linalg.norm(b.reshape(-1, nv) - A @ res[1]["x"].reshape(-1, nv), axis=0)
+    np.testing.assert_allclose(res[1]["abs_res"], tr, rtol=1e-3)
+
+
+def test_true_residual_replacement_rejected_for_merged_bicgstab(O, X):
+    row, col, val = O.cube(8)
+    levels = X.Amg(row, col, val).export()
+    p = mg_params(O, O.JACOBI, O.JACOBI)
+    p.precond = O.role(O.NONE)
+    p.solver = O.role(O.MERGED_BICGSTAB, max_iters=50, rel_tol=1e-8, conv_check=1)
+    p.true_res_period = 2
+    with pytest.raises(RuntimeError):
+        O.solve(O.Hier(levels), p, 1, X.rhs_uniform(0, 8 ** 3, 1))
