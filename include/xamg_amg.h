@@ -58,6 +58,32 @@ int xamg_cube(int64_t nx, int64_t ny, int64_t nz, uint32_t *row, uint32_t *col, 
 /* RHS entries uniform in [-1,1]: b[i] = 2*u(splitmix64(seed, i)) - 1, i = row*nv + k */
 int xamg_rhs_uniform(uint64_t seed, int64_t count, double *b);
 
+/* Variable-coefficient operators of the benchmark configurations (no reference
+ * counterpart beyond the constant-coefficient cube):
+ * k[i] = exp(sigma * z_i), z_i standard normal (counter-based, stream `seed`); k = 1
+ * when sigma = 0 */
+int xamg_lognormal_field(uint64_t seed, double sigma, int64_t count, double *k);
+/* -div(k grad u), cell-centred finite volumes on nx*ny*nz cells, 7-point, harmonic-mean
+ * face coefficients q_d*2k_Pk_Q/(k_P+k_Q), Dirichlet ghost cells (boundary face adds
+ * 2 q_d k_P); row layout and diagonal order of cube.inl:84-146, so k = 1 gives the cube
+ * matrix exactly.  kcell has nx*ny*nz entries. */
+int64_t xamg_varcoef7_nnz(int64_t nx, int64_t ny, int64_t nz);
+int xamg_varcoef7(int64_t nx, int64_t ny, int64_t nz, const double *kcell, uint32_t *row,
+                  uint32_t *col, double *val);
+/* -div(k grad u), trilinear finite elements on the interior nodes of an (nx+1)*(ny+1)*(nz+1)
+ * cell grid of the unit cube (nx*ny*nz unknowns, Dirichlet nodes eliminated), k one value
+ * per cell (kcell has (nx+1)*(ny+1)*(nz+1) entries, x fastest); every existing 27-point
+ * neighbour is stored (face couplings are exactly 0), diagonal first, then ascending
+ * columns; nnz = (3nx-2)(3ny-2)(3nz-2) */
+int64_t xamg_q1fem27_nnz(int64_t nx, int64_t ny, int64_t nz);
+int xamg_q1fem27(int64_t nx, int64_t ny, int64_t nz, const double *kcell, uint32_t *row,
+                 uint32_t *col, double *val);
+/* smooth RHS: b[p*nv+k] = sum of m in [min_bumps, max_bumps] Gaussian bumps
+ * a*exp(-|x-c|^2/(2w^2)) (cell units, centres uniform over the grid, widths uniform in
+ * [min_width, max_width], amplitudes standard normal), cut off below exp(-8) */
+int xamg_rhs_bumps(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
+                   int max_bumps, double min_width, double max_width, double *b);
+
 #ifdef __cplusplus
 }
 #endif

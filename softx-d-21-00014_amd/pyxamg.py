@@ -123,14 +123,19 @@ def _load():
         "xamg_amg_to_device": [vp, P(vp)],
         "xamg_amg_free": [vp],
         "xamg_cube": [i64, i64, i64, vp, vp, vp],
+        "xamg_lognormal_field": [u64, f64, i64, vp],
+        "xamg_varcoef7": [i64, i64, i64, vp, vp, vp, vp],
+        "xamg_q1fem27": [i64, i64, i64, vp, vp, vp, vp],
+        "xamg_rhs_bumps": [u64, i64, i64, i64, C.c_int, C.c_int, C.c_int, f64, f64, vp],
         "xamg_rhs_uniform": [u64, i64, vp],
     }
     for name, args in hsig.items():
         f = getattr(_host, name)
         f.argtypes = args
         f.restype = C.c_int
-    _host.xamg_cube_nnz.argtypes = [i64, i64, i64]
-    _host.xamg_cube_nnz.restype = i64
+    for f in ("xamg_cube_nnz", "xamg_varcoef7_nnz", "xamg_q1fem27_nnz"):
+        getattr(_host, f).argtypes = [i64, i64, i64]
+        getattr(_host, f).restype = i64
     _host.xamg_amg_default_params.argtypes = [P(AmgParams)]
     _host.xamg_amg_default_params.restype = None
     _host.XAMG_get_last_error.restype = C.c_char_p
@@ -171,6 +176,47 @@ def cube(nx, ny=None, nz=None):
     val = np.zeros(nnz, np.float64)
     check(h.xamg_cube(nx, ny, nz, _ptr(row), _ptr(col), _ptr(val)), "xamg_cube")
     return row, col, val
+
+
+def lognormal_field(seed, sigma, count):
+    k = np.zeros(count)
+    check(host().xamg_lognormal_field(seed, sigma, count, _ptr(k)), "xamg_lognormal_field")
+    return k
+
+
+def varcoef7(nx, ny=None, nz=None, sigma=1.0, seed=0, kcell=None):
+    """-div(k grad u), 7-point finite volumes, lognormal cell k: (row, col, val, k)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    k = lognormal_field(seed, sigma, n) if kcell is None else np.ascontiguousarray(kcell, float)
+    nnz = host().xamg_varcoef7_nnz(nx, ny, nz)
+    row, col, val = np.zeros(n + 1, np.uint32), np.zeros(nnz, np.uint32), np.zeros(nnz)
+    check(host().xamg_varcoef7(nx, ny, nz, _ptr(k), _ptr(row), _ptr(col), _ptr(val)),
+          "xamg_varcoef7")
+    return row, col, val, k
+
+
+def q1fem27(nx, ny=None, nz=None, sigma=2.0, seed=0, kcell=None):
+    """-div(k grad u), trilinear finite elements (27-point), lognormal cell k:
+    (row, col, val, k)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    nc = (nx + 1) * (ny + 1) * (nz + 1)
+    k = lognormal_field(seed, sigma, nc) if kcell is None else np.ascontiguousarray(kcell, float)
+    nnz = host().xamg_q1fem27_nnz(nx, ny, nz)
+    row, col, val = np.zeros(n + 1, np.uint32), np.zeros(nnz, np.uint32), np.zeros(nnz)
+    check(host().xamg_q1fem27(nx, ny, nz, _ptr(k), _ptr(row), _ptr(col), _ptr(val)),
+          "xamg_q1fem27")
+    return row, col, val, k
+
+
+def rhs_bumps(seed, nx, ny, nz, nv, min_bumps=8, max_bumps=32, min_width=2.0, max_width=12.0):
+    b = np.zeros(nx * ny * nz * nv)
+    check(host().xamg_rhs_bumps(seed, nx, ny, nz, nv, min_bumps, max_bumps, min_width,
+                                max_width, _ptr(b)), "xamg_rhs_bumps")
+    return b
 
 
 def csr_fold(row, col, ncols=None):

@@ -1,0 +1,46 @@
+"""Variable-coefficient configurations at test size: the 27-point trilinear-FE operator with
+lognormal k (sigma = 2) under AMG + hybrid GS with the FP32 hierarchy, and the 7-point
+finite-volume operator (sigma = 1) under BiCGStab preconditioned by a degree-8 Chebyshev
+polynomial without AMG.  GPU against the oracle on identical matrices and hierarchies:
+iteration counts within 1 and per-RHS relative solution difference <= 1e-6."""
+import numpy as np
+import pytest
+
+from test_gpu_solver import rel_diff, run_both
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mg_prec", [0, 2])
+def test_q1fem27_lognormal_hsgs(gpu, O, mg_prec):
+    X = gpu
+    N, nv = 24, 8
+    row, col, val, _ = X.q1fem27(N, sigma=2.0, seed=1)
+    levels = X.Amg(row, col, val, agg_num_levels=2, num_paths=2, coarse_size=500).export()
+    b = X.rhs_uniform(3, N ** 3, nv)
+    prm = X.default_params(method="MergedPBiCGStab", smoother="HSGS", sweeps=1, max_iters=300,
+                           rel_tol=1e-8, mg_prec=mg_prec)
+    g, r = run_both(X, O, levels, prm, nv, b)
+    assert g["conv"].all() and r["conv"].all()
+    assert abs(g["iters"] - r["iters"]) <= 1
+    assert rel_diff(g["x"], r["x"], nv) <= 1e-6
+    if mg_prec == 2:
+        # the FP32-hierarchy solution against the FP64-hierarchy solve of the same system
+        p64 = X.default_params(method="MergedPBiCGStab", smoother="HSGS", sweeps=1,
+                               max_iters=300, rel_tol=1e-8, mg_prec=0)
+        x64 = X.Solver(X.Hier(levels), p64, nv).solve(b)
+        assert rel_diff(g["x"], x64, nv) <= 1e-6
+
+
+def test_varcoef7_chebyshev8_without_amg(gpu, O):
+    X = gpu
+    N, nv = 20, 4
+    row, col, val, _ = X.varcoef7(N, sigma=1.0, seed=2)
+    levels = X.Amg(row, col, val, max_levels=1).export()
+    b = X.rhs_uniform(4, N ** 3, nv)
+    prm = X.default_params(method="PBiCGStab", precond="Chebyshev", poly_order=8, max_iters=500,
+                           rel_tol=1e-8)
+    g, r = run_both(X, O, levels, prm, nv, b)
+    assert g["conv"].all() and r["conv"].all()
+    assert abs(g["iters"] - r["iters"]) <= 1
+    assert rel_diff(g["x"], r["x"], nv) <= 1e-6
