@@ -29,6 +29,7 @@ extern "C" {
 typedef struct xh_matrix_s *xh_matrix;
 typedef struct xh_hier_s *xh_hier;
 typedef struct xh_solver_s *xh_solver;
+typedef struct xh_batch_s *xh_batch;
 
 /* error codes */
 #define XH_OK 0
@@ -208,6 +209,33 @@ int xh_solver_renew(xh_solver S, const xh_role *solver);
 /* device memory held by the solver (matrices + vectors) */
 int xh_solver_info(xh_solver S, int *nlevels, size_t *device_bytes);
 int xh_solver_destroy(xh_solver S);
+
+/* Batched independent systems (no reference counterpart: the reference solves one system
+ * per call).  nsys CSR systems of different sizes, system s with nv[s] in {1,2,4,8,16}
+ * RHS, solved together by one solver configuration, each exactly as a single solve of
+ * that system: prm->solver BiCGStab, PBiCGStab or MergedPBiCGStab; prm->precond Chebyshev
+ * (order 1..16 on D^-1/2 A D^-1/2, bounds [0, lmax[s]] with the role's spectrum
+ * fraction, chebyshev.inl:50-206) or Identity.  Every iteration is one sequence of
+ * launches over all systems (one hipGraph when prm->graph), each launch a fixed grid
+ * walking (system, row tile) work items; convergence is tracked per system. */
+int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nv,
+                    const uint32_t *const *row, const uint32_t *const *col,
+                    const double *const *val, const double *lmax, const xh_params *prm,
+                    xh_batch *out);
+/* b of system s (n_s x nv_s, interleaved); every solve starts from x = 0 */
+int xh_batch_set_rhs(xh_batch B, int s, const double *b);
+int xh_batch_solve(xh_batch B);
+int xh_batch_get_x(xh_batch B, int s, double *x);
+/* iterations, per-RHS convergence / residuals of system s, and the time from the start of
+ * the last xh_batch_solve until system s finished (device clock, ms) */
+int xh_batch_stats(xh_batch B, int s, int32_t *iters, int32_t *conv, double *abs_res,
+                   double *rel_res, double *latency_ms);
+/* Autotuner: for each nv class present (1, 2, 4, 8, 16) times `reps` batched SpMVs per
+ * candidate work-item size (1..32 passes of 256/nv rows) and keeps the fastest;
+ * tile_passes (nullable) receives the 5 choices.  xh_batch_set_tiles sets them. */
+int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes);
+int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes);
+int xh_batch_destroy(xh_batch B);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,206 @@
+// Batched independent systems: many CSR systems of different sizes and RHS counts solved
+// together, each iteration one sequence of launches over all systems (captured as one
+// hipGraph by the runtime).  A launch walks a list of work items (system, row range);
+// a fixed grid of workgroups takes items blockIdx.x, blockIdx.x + gridDim.x, ..., so
+// uneven systems share one launch.  Inside an item a lane owns (row, RHS) of that
+// system (nv <= 16 lanes per row, 256/nv rows per pass).
+//
+// Each system follows the single-system arithmetic exactly: SpMV row sums in stored
+// order without FMA contraction (csr_Ax_y, src/blas2/detail/csr_matvec.inl:149-260), the
+// Chebyshev steps of chebyshev.inl:179-196 and the BiCGStab updates of
+// src/solvers/detail/pbicgstab.inl:141-230; dot partials go one block per item and are
+// reduced per system, in item order, by finalize_body.
+#include "xh_internal.hpp"
+#include "xh_krylov.hpp"
+
+#include <algorithm>
+
+namespace xh {
+namespace {
+
+// per-item reduction of nd dots over the rows of the item, fixed order
+__device__ __forceinline__ void item_partials(const double (&part)[3], int nd, int nv,
+                                              double *__restrict__ out) {
+    __shared__ double red[3][kBlock];
+    const int tid = threadIdx.x;
+    for (int d = 0; d < nd; ++d)
+        red[d][tid] = part[d];
+    __syncthreads();
+    if (tid < nd * nv) {
+        const int d = tid / nv, k = tid % nv, rpp = kBlock / nv;
+        double s = red[d][k];
+        for (int q = 1; q < rpp; ++q)
+            s = s + red[d][q * nv + k];
+        out[d * nv + k] = s;
+    }
+    __syncthreads();
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_batch(BatchDev d, const BatchItem *__restrict__ items,
+                                                  int nitems, BatchVecs v, int ci, int last,
+                                                  double *__restrict__ partials) {
+    constexpr int ND = (OP == B_RESID0 || OP == B_DOT1 || OP == B_XR_UPDATE) ? 1
+                       : (OP == B_DOT3)                                      ? 3
+                                                                             : 0;
+    const int tid = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchItem w = items[it];
+        const SolverState *st = d.st + w.sys;
+        if (st->done)
+            continue; // uniform over the workgroup: one item per workgroup at a time
+        const int nv = w.nv, rpp = kBlock / nv;
+        const int tr = tid / nv, k = tid % nv;
+        const int64_t gb = d.rbase[w.sys];
+        const size_t vo = (size_t)d.voff[w.sys];
+        double part[3] = {0.0, 0.0, 0.0};
+        double c1 = 0, c2 = 0, c3 = 0;
+        if constexpr (OP == B_S_UPDATE) {
+            c1 = st->nalpha[k];
+        } else if constexpr (OP == B_XR_UPDATE) {
+            c1 = st->ac[k];
+            c2 = st->oc[k];
+            c3 = st->omega[k];
+        } else if constexpr (OP == B_P_UPDATE) {
+            c1 = st->pcoef[k];
+            c2 = st->beta[k];
+        } else if constexpr (OP == B_CHEB0Z || OP == B_CHEB) {
+            c1 = d.cheb[(size_t)w.sys * kChebMax + ci];
+        }
+        if (tr < rpp)
+            for (int64_t i = w.r0 + tr; i < w.r1; i += rpp) {
+                const size_t o = vo + (size_t)i * nv + k;
+                const int64_t g = gb + i;
+                if constexpr (OP == B_RESID0 || OP == B_DOT1 || OP == B_DOT3 || OP == B_CHEB) {
+                    const double *xs = v.in + vo;
+                    double acc = 0.0;
+                    for (uint32_t j = d.row[g]; j < d.row[g + 1]; ++j)
+                        acc = acc + xs[(size_t)d.col[j] * nv + k] * d.val[j];
+                    if constexpr (OP == B_RESID0) {
+                        // r0 = b - A x ; p = r = r0 ; (r0, r0)
+                        const double t = v.b[o] - acc;
+                        part[0] = part[0] + t * t;
+                        v.out[o] = t;
+                        v.out2[o] = t;
+                        v.out3[o] = t;
+                    } else if constexpr (OP == B_DOT1) {
+                        v.out[o] = acc;
+                        part[0] = part[0] + acc * v.o1[o];
+                    } else if constexpr (OP == B_DOT3) {
+                        const double sv = v.o1[o];
+                        v.out[o] = acc;
+                        part[0] = part[0] + acc * sv;
+                        part[1] = part[1] + acc * acc;
+                        part[2] = part[2] + sv * sv;
+                    } else { // B_CHEB: t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
+                        const double di = d.isd[g];
+                        const double t2 = acc * di;
+                        double t1 = c1 * v.o1[o] + t2;
+                        t1 = t1 * di;
+                        if (last)
+                            v.out2[o] = t1; // x is zero-flagged: x = t1
+                        else
+                            v.out[o] = t1;
+                    }
+                } else if constexpr (OP == B_CHEB0Z) {
+                    // residual of a zero-flagged x is b: r = b*isd ; t1 = (c*r)*isd
+                    const double di = d.isd[g];
+                    const double rv = v.b[o] * di;
+                    double t1 = c1 * rv;
+                    t1 = t1 * di;
+                    if (last) {
+                        v.out2[o] = t1;
+                    } else {
+                        v.out3[o] = rv;
+                        v.out[o] = t1;
+                    }
+                } else if constexpr (OP == B_COPY) {
+                    v.out[o] = v.b[o];
+                } else if constexpr (OP == B_S_UPDATE) {
+                    v.out[o] = v.b[o] + c1 * v.o1[o]; // s = r + (-alpha) v
+                } else if constexpr (OP == B_XR_UPDATE) {
+                    // x = ac p_ + oc s_ + x ; r = s - omega t ; (r, r0)
+                    v.x[o] = c1 * v.p_[o] + c2 * v.s_[o] + v.x[o];
+                    const double rv = v.s[o] - c3 * v.t[o];
+                    v.out[o] = rv;
+                    part[0] = part[0] + rv * v.o1[o];
+                } else if constexpr (OP == B_P_UPDATE) {
+                    // p = r + (-(omega beta)) v + beta p
+                    v.out[o] = v.b[o] + c1 * v.o1[o] + c2 * v.out[o];
+                }
+            }
+        if constexpr (ND > 0)
+            item_partials(part, ND, nv, partials + (size_t)it * kBatchPart);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_finalize(int op, SolverState *__restrict__ st,
+                                                        const double *__restrict__ partials,
+                                                        const int *__restrict__ item_start,
+                                                        const int *__restrict__ item_count,
+                                                        const int *__restrict__ nvs) {
+    const int s = blockIdx.x;
+    finalize_body(op, st + s, partials + (size_t)item_start[s] * kBatchPart, item_count[s],
+                  nvs[s], kBatchPart);
+}
+
+// flag[0] = 1 when every system is done; flag[1] = number of systems not done
+__global__ __launch_bounds__(256) void k_batch_all_done(const SolverState *__restrict__ st,
+                                                        int nsys, int *__restrict__ flag) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0)
+        cnt = 0;
+    __syncthreads();
+    int c = 0;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x)
+        c += st[s].done ? 0 : 1;
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        flag[0] = cnt == 0;
+        flag[1] = cnt;
+    }
+}
+
+__global__ void k_stamp(unsigned long long *t) { *t = wall_clock64(); }
+
+} // namespace
+
+void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
+                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s) {
+    if (nitems == 0)
+        return;
+    const dim3 g(std::max(1, std::min(grid, nitems))), b(kBlock);
+#define XH_BATCH_CASE(OPV)                                                                       \
+    case OPV:                                                                                    \
+        hipLaunchKernelGGL(k_batch<OPV>, g, b, 0, s, d, items, nitems, v, ci, last, partials);   \
+        break;
+    switch (op) {
+        XH_BATCH_CASE(B_RESID0)
+        XH_BATCH_CASE(B_DOT1)
+        XH_BATCH_CASE(B_DOT3)
+        XH_BATCH_CASE(B_CHEB0Z)
+        XH_BATCH_CASE(B_CHEB)
+        XH_BATCH_CASE(B_COPY)
+        XH_BATCH_CASE(B_S_UPDATE)
+        XH_BATCH_CASE(B_XR_UPDATE)
+        XH_BATCH_CASE(B_P_UPDATE)
+    }
+#undef XH_BATCH_CASE
+}
+
+void batch_finalize(FinOp op, SolverState *st, const double *partials, const int *item_start,
+                    const int *item_count, const int *nvs, int nsys, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_finalize, dim3(nsys), dim3(256), 0, s, (int)op, st, partials,
+                       item_start, item_count, nvs);
+}
+
+void batch_all_done(const SolverState *st, int nsys, int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_all_done, dim3(1), dim3(256), 0, s, st, nsys, flag);
+}
+
+void stamp_clock(unsigned long long *t, hipStream_t s) {
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, t);
+}
+
+} // namespace xh

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 namespace xh {
 
@@ -116,6 +117,7 @@ struct SolverState {
     int nv, max_iters, conv_check, hist_len;
     float abs_tol, rel_tol;
     double *hist; // [hist_len][nv] sqrt(res) per convergence check, or nullptr
+    unsigned long long t_done; // device wall clock (wall_clock64) when `done` was set
 };
 
 // ---- SpMV family (xh_spmv.hip) ----
@@ -189,6 +191,64 @@ void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int
 
 // smoother / transfer helpers
 // `done` (nullable): device flag; when set the launch is a no-op
+// error message for xh_last_error(); returns `code`
+int report_error(int code, const std::string &msg);
+// selects device 0 on first use
+int runtime_init();
+
+// ---- batched independent systems (xh_batch.hip) ----
+constexpr int kChebMax = 16;          // Chebyshev coefficients per system
+constexpr int kBatchPart = 3 * 16;    // dot partial stride per work item (3 dots x nv <= 16)
+constexpr int kBatchMaxNV = 16;
+
+struct BatchItem {        // rows [r0, r1) of system `sys`, which has nv RHS
+    int32_t sys, nv;
+    int64_t r0, r1;
+};
+
+struct BatchDev {
+    const uint32_t *row = nullptr;   // concatenated row pointers (global nonzero offsets)
+    const uint32_t *col = nullptr;   // columns local to the system
+    const double *val = nullptr;
+    const double *isd = nullptr;     // inverse square-root diagonal per global row
+    const int64_t *rbase = nullptr;  // first global row of each system
+    const int64_t *voff = nullptr;   // first multivector element of each system
+    SolverState *st = nullptr;       // one state per system
+    const double *cheb = nullptr;    // kChebMax coefficients per system
+};
+
+// multivector operands of one batched launch (all in the concatenated layout)
+struct BatchVecs {
+    const double *in = nullptr;  // SpMV operand
+    const double *b = nullptr;   // RESID0, CHEB0Z: b; COPY: source; S_/P_UPDATE: r
+    const double *o1 = nullptr;  // DOT1, XR_UPDATE: r0; DOT3: s; CHEB: scaled r; S_/P_UPDATE: v
+    double *out = nullptr;       // result (P_UPDATE: p, read and written)
+    double *out2 = nullptr;      // RESID0: p; CHEB0Z/CHEB with `last`: x
+    double *out3 = nullptr;      // RESID0: r; CHEB0Z: scaled r
+    double *x = nullptr;         // XR_UPDATE
+    const double *p_ = nullptr, *s_ = nullptr, *s = nullptr, *t = nullptr; // XR_UPDATE
+};
+
+enum BatchOp : int {
+    B_RESID0 = 0,    // out = b - A in ; out2 = out3 = out ; (out, out)
+    B_DOT1 = 1,      // out = A in ; (out, o1)
+    B_DOT3 = 2,      // out = A in ; (out, o1), (out, out), (o1, o1)
+    B_CHEB0Z = 3,    // first Chebyshev step from a zero x (coefficient ci)
+    B_CHEB = 4,      // one Horner step (coefficient ci), x = t1*isd when `last`
+    B_COPY = 5,      // out = b
+    B_S_UPDATE = 6,  // out = b + nalpha o1
+    B_XR_UPDATE = 7, // x = ac p_ + oc s_ + x ; out = s - omega t ; (out, o1)
+    B_P_UPDATE = 8,  // out = b + pcoef o1 + beta out
+};
+
+void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
+                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s);
+void batch_finalize(FinOp op, SolverState *st, const double *partials, const int *item_start,
+                    const int *item_count, const int *nvs, int nsys, hipStream_t s);
+// flag[0] = every system done, flag[1] = systems not done
+void batch_all_done(const SolverState *st, int nsys, int *flag, hipStream_t s);
+void stamp_clock(unsigned long long *t, hipStream_t s);
+
 // x_c[c][k] = x_full[col_ind[c] + off][k], c < nc (the compressed block's column decode)
 void gather_cols(int64_t nc, int nv, int prec, const uint32_t *col_ind, int64_t off,
                  const void *x_full, void *x_c, hipStream_t s);

@@ -55,6 +55,14 @@ int ensure_init() {
     return xh_init(0);
 }
 
+} // namespace
+
+// shared with the other sources of the library (xh_internal.hpp)
+int report_error(int code, const std::string &msg) { return fail(code, msg); }
+int runtime_init() { return ensure_init(); }
+
+namespace {
+
 // ---- host CSR ----
 struct HostCsr {
     int64_t nrows = 0, ncols = 0;
@@ -1493,5 +1501,547 @@ extern "C" int xh_matrix_gather(xh_matrix A, int nv, const void *dx_full, void *
                 g_stream);
     XH_HIP(hipGetLastError());
     XH_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched independent systems
+// ---------------------------------------------------------------------------------------
+
+struct xh_batch_s {
+    int nsys = 0;
+    xh_params prm;
+    std::vector<int64_t> n, rbase, voff;
+    std::vector<int> nv;
+    int64_t rows = 0, nnz = 0, velems = 0;
+    hipStream_t stream = nullptr;
+    // device
+    uint32_t *row = nullptr, *col = nullptr;
+    double *val = nullptr, *isd = nullptr, *cheb = nullptr;
+    int64_t *d_rbase = nullptr, *d_voff = nullptr;
+    int *d_nv = nullptr, *d_item_start = nullptr, *d_item_count = nullptr, *d_flag = nullptr;
+    BatchItem *d_items = nullptr;
+    SolverState *st = nullptr;
+    double *partials = nullptr;
+    double *x = nullptr, *b = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr;
+    double *s = nullptr, *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cr = nullptr,
+           *cu1 = nullptr, *cu2 = nullptr;
+    unsigned long long *d_t0 = nullptr;
+    // host
+    std::vector<BatchItem> items;
+    std::vector<int> item_start, item_count;
+    int tile_passes[5] = {8, 8, 8, 8, 8}; // per nv class 1, 2, 4, 8, 16
+    int grid = 2048;
+    SolverState *hst = nullptr; // pinned [nsys]
+    int *hflag = nullptr;       // pinned [2][2]
+    unsigned long long *ht0 = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double wall_khz = 0.0;
+    size_t bytes = 0;
+};
+
+namespace xh {
+namespace {
+
+int nv_class(int nv) { return nv == 1 ? 0 : nv == 2 ? 1 : nv == 4 ? 2 : nv == 8 ? 3 : 4; }
+
+// work items of the systems whose nv class is in `mask`, tile_passes[class] passes of
+// 256/nv rows each; item_start/count filled when all classes are included
+void build_items(xh_batch_s *B, unsigned mask, std::vector<BatchItem> &items,
+                 std::vector<int> *start, std::vector<int> *count) {
+    items.clear();
+    if (start) {
+        start->assign(B->nsys, 0);
+        count->assign(B->nsys, 0);
+    }
+    for (int s = 0; s < B->nsys; ++s) {
+        const int c = nv_class(B->nv[s]);
+        if (!(mask & (1u << c)))
+            continue;
+        const int64_t tile = (int64_t)B->tile_passes[c] * (kBlock / B->nv[s]);
+        if (start)
+            (*start)[s] = (int)items.size();
+        for (int64_t r0 = 0; r0 < B->n[s]; r0 += tile)
+            items.push_back({s, B->nv[s], r0, std::min(B->n[s], r0 + tile)});
+        if (count)
+            (*count)[s] = (int)items.size() - (*start)[s];
+    }
+}
+
+int upload_items(xh_batch_s *B) {
+    build_items(B, 0x1f, B->items, &B->item_start, &B->item_count);
+    (void)hipFree(B->d_items);
+    (void)hipFree(B->partials);
+    B->d_items = nullptr;
+    B->partials = nullptr;
+    size_t acc = 0;
+    int rc;
+    if ((rc = dmalloc(&B->d_items, B->items.size(), acc)) ||
+        (rc = dmalloc(&B->partials, B->items.size() * kBatchPart, acc)))
+        return rc;
+    XH_HIP(hipMemcpy(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
+                     hipMemcpyHostToDevice));
+    XH_HIP(hipMemcpy(B->d_item_start, B->item_start.data(), B->nsys * sizeof(int),
+                     hipMemcpyHostToDevice));
+    XH_HIP(hipMemcpy(B->d_item_count, B->item_count.data(), B->nsys * sizeof(int),
+                     hipMemcpyHostToDevice));
+    if (B->gexec) {
+        (void)hipGraphExecDestroy(B->gexec);
+        (void)hipGraphDestroy(B->graph);
+        B->gexec = nullptr;
+        B->graph = nullptr;
+    }
+    return 0;
+}
+
+BatchDev batch_dev(const xh_batch_s *B) {
+    BatchDev d;
+    d.row = B->row;
+    d.col = B->col;
+    d.val = B->val;
+    d.isd = B->isd;
+    d.rbase = B->d_rbase;
+    d.voff = B->d_voff;
+    d.st = B->st;
+    d.cheb = B->cheb;
+    return d;
+}
+
+// x = M^-1 b from a zero x: Chebyshev polynomial (smooth() with x_zero) or a copy
+void batch_precond(xh_batch_s *B, const double *b, double *x) {
+    const BatchDev d = batch_dev(B);
+    const int nit = (int)B->items.size();
+    hipStream_t st = B->stream;
+    if (B->prm.precond.method == XH_CHEBYSHEV) {
+        const int k = B->prm.precond.poly_order;
+        BatchVecs v;
+        v.b = b;
+        v.out = k == 1 ? x : B->cu1;
+        v.out2 = x;
+        v.out3 = B->cr;
+        batch_launch(B_CHEB0Z, d, B->d_items, nit, B->grid, v, k - 1, k == 1, nullptr, st);
+        double *uc = B->cu1, *un = B->cu2;
+        for (int i = k - 2; i >= 0; --i) {
+            BatchVecs w;
+            w.in = uc;
+            w.o1 = B->cr;
+            w.out = un;
+            w.out2 = x;
+            batch_launch(B_CHEB, d, B->d_items, nit, B->grid, w, i, i == 0, nullptr, st);
+            std::swap(uc, un);
+        }
+        return;
+    }
+    BatchVecs v;
+    v.b = b;
+    v.out = x;
+    batch_launch(B_COPY, d, B->d_items, nit, B->grid, v, 0, 0, nullptr, st);
+}
+
+void batch_iteration(xh_batch_s *B) {
+    const BatchDev d = batch_dev(B);
+    const int nit = (int)B->items.size();
+    hipStream_t st = B->stream;
+    const bool pre = B->prm.solver.method != XH_BICGSTAB;
+    double *pp = pre ? B->p_ : B->p, *ss = pre ? B->s_ : B->s;
+    auto fin = [&](FinOp op) {
+        batch_finalize(op, B->st, B->partials, B->d_item_start, B->d_item_count, B->d_nv,
+                       B->nsys, st);
+    };
+    if (pre)
+        batch_precond(B, B->p, B->p_);
+    BatchVecs v1;
+    v1.in = pp;
+    v1.o1 = B->r0;
+    v1.out = B->v;
+    batch_launch(B_DOT1, d, B->d_items, nit, B->grid, v1, 0, 0, B->partials, st);
+    fin(FIN_ALPHA);
+    BatchVecs vs;
+    vs.b = B->r;
+    vs.o1 = B->v;
+    vs.out = B->s;
+    batch_launch(B_S_UPDATE, d, B->d_items, nit, B->grid, vs, 0, 0, nullptr, st);
+    if (pre)
+        batch_precond(B, B->s, B->s_);
+    BatchVecs v3;
+    v3.in = ss;
+    v3.o1 = B->s;
+    v3.out = B->t;
+    batch_launch(B_DOT3, d, B->d_items, nit, B->grid, v3, 0, 0, B->partials, st);
+    fin(FIN_OMEGA);
+    BatchVecs vx;
+    vx.x = B->x;
+    vx.p_ = pp;
+    vx.s_ = ss;
+    vx.s = B->s;
+    vx.t = B->t;
+    vx.o1 = B->r0;
+    vx.out = B->r;
+    batch_launch(B_XR_UPDATE, d, B->d_items, nit, B->grid, vx, 0, 0, B->partials, st);
+    fin(FIN_BETA);
+    BatchVecs vp;
+    vp.b = B->r;
+    vp.o1 = B->v;
+    vp.out = B->p;
+    batch_launch(B_P_UPDATE, d, B->d_items, nit, B->grid, vp, 0, 0, nullptr, st);
+    batch_all_done(B->st, B->nsys, B->d_flag, st);
+}
+
+void free_batch(xh_batch_s *B) {
+    if (B->stream)
+        (void)hipStreamSynchronize(B->stream);
+    if (B->gexec)
+        (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph)
+        (void)hipGraphDestroy(B->graph);
+    for (void *q : {(void *)B->row, (void *)B->col, (void *)B->val, (void *)B->isd,
+                    (void *)B->cheb, (void *)B->d_rbase, (void *)B->d_voff, (void *)B->d_nv,
+                    (void *)B->d_item_start, (void *)B->d_item_count, (void *)B->d_flag,
+                    (void *)B->d_items, (void *)B->st, (void *)B->partials, (void *)B->x,
+                    (void *)B->b, (void *)B->r0, (void *)B->r, (void *)B->p, (void *)B->v,
+                    (void *)B->s, (void *)B->t, (void *)B->p_, (void *)B->s_, (void *)B->cr,
+                    (void *)B->cu1, (void *)B->cu2, (void *)B->d_t0})
+        if (q)
+            (void)hipFree(q);
+    if (B->hst)
+        (void)hipHostFree(B->hst);
+    if (B->hflag)
+        (void)hipHostFree(B->hflag);
+    if (B->ht0)
+        (void)hipHostFree(B->ht0);
+    for (hipEvent_t e : B->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (B->stream)
+        (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+} // namespace
+} // namespace xh
+
+extern "C" int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nvs,
+                               const uint32_t *const *row, const uint32_t *const *col,
+                               const double *const *val, const double *lmax,
+                               const xh_params *prm, xh_batch *out) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (nsys < 1 || !nrows || !nvs || !row || !col || !val || !prm || !out)
+        return fail(XH_EINVAL, "xh_batch_create: bad arguments");
+    const int m = prm->solver.method;
+    if (m != XH_BICGSTAB && m != XH_PBICGSTAB && m != XH_MERGED_PBICGSTAB)
+        return fail(XH_ENOTSUP, "xh_batch_create: solver must be BiCGStab, PBiCGStab or "
+                                "MergedPBiCGStab");
+    const int pm = prm->precond.method;
+    if (m != XH_BICGSTAB && pm != XH_CHEBYSHEV && pm != XH_IDENTITY)
+        return fail(XH_ENOTSUP, "xh_batch_create: batched preconditioner must be Chebyshev "
+                                "or Identity");
+    if (m != XH_BICGSTAB && pm == XH_CHEBYSHEV &&
+        (prm->precond.poly_order < 1 || prm->precond.poly_order > kChebMax || !lmax))
+        return fail(XH_EINVAL, "xh_batch_create: Chebyshev order 1..16 and lmax needed");
+    auto *B = new xh_batch_s();
+    B->nsys = nsys;
+    B->prm = *prm;
+    int64_t rows = 0, nnz = 0, ve = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const int v = nvs[s];
+        if (!(v == 1 || v == 2 || v == 4 || v == 8 || v == 16) || nrows[s] < 1 || !row[s] ||
+            row[s][0] != 0) {
+            free_batch(B);
+            return fail(XH_EINVAL, "xh_batch_create: system " + std::to_string(s) +
+                                       ": nv must be 1, 2, 4, 8 or 16 and rows >= 1");
+        }
+        const int64_t n = nrows[s], z = row[s][n];
+        for (int64_t i = 0; i < n; ++i)
+            if (row[s][i + 1] < row[s][i]) {
+                free_batch(B);
+                return fail(XH_EINVAL, "xh_batch_create: row pointer not monotone");
+            }
+        for (int64_t j = 0; j < z; ++j)
+            if ((int64_t)col[s][j] >= n) {
+                free_batch(B);
+                return fail(XH_EINVAL, "xh_batch_create: column index out of range");
+            }
+        B->n.push_back(n);
+        B->nv.push_back(v);
+        B->rbase.push_back(rows);
+        B->voff.push_back(ve);
+        rows += n;
+        nnz += z;
+        ve += n * v;
+    }
+    if (nnz >= ((int64_t)1 << 32)) {
+        free_batch(B);
+        return fail(XH_EINVAL, "xh_batch_create: more than 2^32-1 nonzeros in the batch");
+    }
+    B->rows = rows;
+    B->nnz = nnz;
+    B->velems = ve;
+    // concatenated CSR, inverse square-root diagonals, Chebyshev coefficients
+    std::vector<uint32_t> hrow(rows + 1), hcol(nnz);
+    std::vector<double> hval(nnz), hisd(rows, 0.0), hcheb((size_t)nsys * kChebMax, 0.0);
+    int64_t zr = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const int64_t n = B->n[s], g0 = B->rbase[s];
+        for (int64_t i = 0; i < n; ++i) {
+            hrow[g0 + i] = (uint32_t)(zr + row[s][i]);
+            for (uint32_t j = row[s][i]; j < row[s][i + 1]; ++j)
+                if (col[s][j] == (uint32_t)i)
+                    hisd[g0 + i] = std::sqrt(1.0 / val[s][j]);
+        }
+        std::copy(col[s], col[s] + row[s][n], hcol.begin() + zr);
+        std::copy(val[s], val[s] + row[s][n], hval.begin() + zr);
+        zr += row[s][n];
+        if (m != XH_BICGSTAB && pm == XH_CHEBYSHEV) {
+            std::vector<double> c;
+            cheby_coeffs(0.0, lmax[s], prm->precond.poly_order,
+                         (double)prm->precond.spectrum_fraction, c);
+            std::copy(c.begin(), c.end(), hcheb.begin() + (size_t)s * kChebMax);
+        }
+    }
+    hrow[rows] = (uint32_t)zr;
+    size_t &acc = B->bytes;
+    auto bail = [&](int code) {
+        free_batch(B);
+        return code;
+    };
+    if ((rc = dmalloc(&B->row, rows + 1, acc)) || (rc = dmalloc(&B->col, std::max<int64_t>(1, nnz), acc)) ||
+        (rc = dmalloc(&B->val, std::max<int64_t>(1, nnz), acc)) || (rc = dmalloc(&B->isd, rows, acc)) ||
+        (rc = dmalloc(&B->cheb, (size_t)nsys * kChebMax, acc)) ||
+        (rc = dmalloc(&B->d_rbase, nsys, acc)) || (rc = dmalloc(&B->d_voff, nsys, acc)) ||
+        (rc = dmalloc(&B->d_nv, nsys, acc)) || (rc = dmalloc(&B->d_item_start, nsys, acc)) ||
+        (rc = dmalloc(&B->d_item_count, nsys, acc)) || (rc = dmalloc(&B->d_flag, 2, acc)) ||
+        (rc = dmalloc(&B->st, nsys, acc)) || (rc = dmalloc(&B->d_t0, 1, acc)))
+        return bail(rc);
+    for (double **q : {&B->x, &B->b, &B->r0, &B->r, &B->p, &B->v, &B->s, &B->t, &B->p_, &B->s_,
+                       &B->cr, &B->cu1, &B->cu2})
+        if ((rc = dmalloc(q, ve, acc)))
+            return bail(rc);
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes)
+            e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    cp(B->row, hrow.data(), (rows + 1) * 4);
+    cp(B->col, hcol.data(), nnz * 4);
+    cp(B->val, hval.data(), nnz * 8);
+    cp(B->isd, hisd.data(), rows * 8);
+    cp(B->cheb, hcheb.data(), hcheb.size() * 8);
+    cp(B->d_rbase, B->rbase.data(), nsys * 8);
+    cp(B->d_voff, B->voff.data(), nsys * 8);
+    cp(B->d_nv, B->nv.data(), nsys * 4);
+    if (e == hipSuccess)
+        e = hipMemset(B->b, 0, ve * 8);
+    if (e == hipSuccess)
+        e = hipStreamCreate(&B->stream);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&B->hst, sizeof(SolverState) * nsys);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&B->hflag, sizeof(int) * 4);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&B->ht0, sizeof(unsigned long long));
+    for (hipEvent_t &ev : B->ev)
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    int khz = 0;
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g_device);
+    if (e != hipSuccess)
+        return bail(fail(XH_EHIP, std::string("xh_batch_create: ") + hipGetErrorString(e)));
+    B->wall_khz = khz;
+    if ((rc = upload_items(B)))
+        return bail(rc);
+    *out = B;
+    return 0;
+}
+
+extern "C" int xh_batch_set_rhs(xh_batch B, int s, const double *b) {
+    if (!B || s < 0 || s >= B->nsys || !b)
+        return fail(XH_EINVAL, "xh_batch_set_rhs: bad arguments");
+    XH_HIP(hipMemcpy(B->b + B->voff[s], b, B->n[s] * B->nv[s] * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int xh_batch_get_x(xh_batch B, int s, double *x) {
+    if (!B || s < 0 || s >= B->nsys || !x)
+        return fail(XH_EINVAL, "xh_batch_get_x: bad arguments");
+    XH_HIP(hipMemcpy(x, B->x + B->voff[s], B->n[s] * B->nv[s] * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int xh_batch_solve(xh_batch B) {
+    if (!B)
+        return fail(XH_EINVAL, "xh_batch_solve: null batch");
+    const int nsys = B->nsys;
+    for (int s = 0; s < nsys; ++s) {
+        SolverState &h = B->hst[s];
+        std::memset(&h, 0, sizeof(SolverState));
+        for (int k = 0; k < B->nv[s]; ++k)
+            h.iconv[k] = 1.0;
+        h.nv = B->nv[s];
+        h.max_iters = B->prm.solver.max_iters;
+        h.conv_check = B->prm.solver.conv_check;
+        h.abs_tol = B->prm.solver.abs_tol;
+        h.rel_tol = B->prm.solver.rel_tol;
+    }
+    hipStream_t st = B->stream;
+    XH_HIP(hipMemcpyAsync(B->st, B->hst, sizeof(SolverState) * nsys, hipMemcpyHostToDevice, st));
+    fill_zero(B->velems, F64, B->x, nullptr, st);
+    stamp_clock(B->d_t0, st);
+    // r0 = b - A x ; p = r = r0 ; rho0 (get_residual + copies, pbicgstab.inl:111-117)
+    BatchVecs v;
+    v.in = B->x;
+    v.b = B->b;
+    v.out = B->r0;
+    v.out2 = B->p;
+    v.out3 = B->r;
+    const BatchDev d = batch_dev(B);
+    batch_launch(B_RESID0, d, B->d_items, (int)B->items.size(), B->grid, v, 0, 0, B->partials, st);
+    batch_finalize(FIN_INIT, B->st, B->partials, B->d_item_start, B->d_item_count, B->d_nv,
+                   nsys, st);
+    int rc = check_launch();
+    if (rc)
+        return rc;
+    const bool use_graph = B->prm.graph != 0;
+    if (use_graph && B->gexec == nullptr) {
+        XH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        batch_iteration(B);
+        XH_HIP(hipStreamEndCapture(st, &B->graph));
+        XH_HIP(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+    }
+    const int maxit = std::max(1, B->prm.solver.max_iters) + 1;
+    for (int it = 0; it < maxit; ++it) {
+        if (use_graph) {
+            XH_HIP(hipGraphLaunch(B->gexec, st));
+        } else {
+            batch_iteration(B);
+            if ((rc = check_launch()))
+                return rc;
+        }
+        XH_HIP(hipMemcpyAsync(&B->hflag[2 * (it & 1)], B->d_flag, 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+        XH_HIP(hipEventRecord(B->ev[it & 1], st));
+        if (it > 0) {
+            XH_HIP(hipEventSynchronize(B->ev[(it - 1) & 1]));
+            if (B->hflag[2 * ((it - 1) & 1)])
+                break;
+        }
+    }
+    XH_HIP(hipMemcpyAsync(B->hst, B->st, sizeof(SolverState) * nsys, hipMemcpyDeviceToHost, st));
+    XH_HIP(hipMemcpyAsync(B->ht0, B->d_t0, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    XH_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < nsys; ++s)
+        if (!B->hst[s].done)
+            return fail(XH_EINVAL, "xh_batch_solve: system " + std::to_string(s) +
+                                       " did not terminate within max_iters");
+    return 0;
+}
+
+extern "C" int xh_batch_stats(xh_batch B, int s, int32_t *iters, int32_t *conv,
+                              double *abs_res, double *rel_res, double *latency_ms) {
+    if (!B || s < 0 || s >= B->nsys)
+        return fail(XH_EINVAL, "xh_batch_stats: bad arguments");
+    const SolverState &h = B->hst[s];
+    if (iters)
+        *iters = h.iters;
+    for (int k = 0; k < B->nv[s]; ++k) {
+        if (conv)
+            conv[k] = h.conv[k];
+        if (abs_res)
+            abs_res[k] = h.abs_res[k];
+        if (rel_res)
+            rel_res[k] = h.rel_res[k];
+    }
+    if (latency_ms)
+        *latency_ms = B->wall_khz > 0 ? (double)(h.t_done - *B->ht0) / B->wall_khz : 0.0;
+    return 0;
+}
+
+extern "C" int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes) {
+    if (!B || reps < 1)
+        return fail(XH_EINVAL, "xh_batch_autotune: bad arguments");
+    static const int cand[] = {1, 2, 4, 8, 16, 32};
+    hipStream_t st = B->stream;
+    hipEvent_t e0, e1;
+    XH_HIP(hipEventCreate(&e0));
+    XH_HIP(hipEventCreate(&e1));
+    // every system "not done" so the timed launches do their work
+    for (int s = 0; s < B->nsys; ++s) {
+        std::memset(&B->hst[s], 0, sizeof(SolverState));
+        B->hst[s].nv = B->nv[s];
+    }
+    XH_HIP(hipMemcpy(B->st, B->hst, sizeof(SolverState) * B->nsys, hipMemcpyHostToDevice));
+    std::vector<BatchItem> items;
+    BatchItem *d_items = nullptr;
+    double *d_part = nullptr;
+    int rc = 0;
+    for (int c = 0; c < 5 && rc == 0; ++c) {
+        bool present = false;
+        for (int s = 0; s < B->nsys; ++s)
+            present = present || nv_class(B->nv[s]) == c;
+        if (!present)
+            continue;
+        double best = 1e30;
+        int best_p = B->tile_passes[c];
+        for (int p : cand) {
+            B->tile_passes[c] = p;
+            build_items(B, 1u << c, items, nullptr, nullptr);
+            (void)hipFree(d_items);
+            (void)hipFree(d_part);
+            size_t acc = 0;
+            if ((rc = dmalloc(&d_items, items.size(), acc)) ||
+                (rc = dmalloc(&d_part, items.size() * kBatchPart, acc)))
+                break;
+            XH_HIP(hipMemcpy(d_items, items.data(), items.size() * sizeof(BatchItem),
+                             hipMemcpyHostToDevice));
+            BatchVecs v;
+            v.in = B->b;
+            v.o1 = B->b;
+            v.out = B->t;
+            const BatchDev d = batch_dev(B);
+            batch_launch(B_DOT1, d, d_items, (int)items.size(), B->grid, v, 0, 0, d_part, st);
+            XH_HIP(hipEventRecord(e0, st));
+            for (int q = 0; q < reps; ++q)
+                batch_launch(B_DOT1, d, d_items, (int)items.size(), B->grid, v, 0, 0, d_part, st);
+            XH_HIP(hipEventRecord(e1, st));
+            XH_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            XH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_p = p;
+            }
+        }
+        B->tile_passes[c] = best_p;
+    }
+    (void)hipFree(d_items);
+    (void)hipFree(d_part);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc)
+        return rc;
+    if (tile_passes)
+        for (int c = 0; c < 5; ++c)
+            tile_passes[c] = B->tile_passes[c];
+    return upload_items(B);
+}
+
+extern "C" int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes) {
+    if (!B || !tile_passes)
+        return fail(XH_EINVAL, "xh_batch_set_tiles: bad arguments");
+    for (int c = 0; c < 5; ++c)
+        if (tile_passes[c] < 1 || tile_passes[c] > 1024)
+            return fail(XH_EINVAL, "xh_batch_set_tiles: tile passes must be 1..1024");
+    for (int c = 0; c < 5; ++c)
+        B->tile_passes[c] = tile_passes[c];
+    return upload_items(B);
+}
+
+extern "C" int xh_batch_destroy(xh_batch B) {
+    if (B)
+        free_batch(B);
     return 0;
 }

@@ -13,6 +13,7 @@
 // Every expression keeps the reference's operand order; the file is built with
 // -ffp-contract=off so a*b+c rounds twice as in the reference's x86-64 build.
 #include "xh_internal.hpp"
+#include "xh_krylov.hpp"
 
 #include <algorithm>
 
@@ -168,148 +169,10 @@ int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
 
 // ---- finalize: partial sums -> Krylov scalars and the convergence decision ----
 
-__device__ void conv_check(SolverState *st, const double *res, const double *res0, int k,
-                           int nv, int *s_notconv) {
-    // base_solver::convergence_check (solver.h:173-203), tolerances are float32
-    if (k < nv && st->iconv[k] != 0.0) {
-        const double a = sqrt(res[k]);
-        const double rr = a / sqrt(res0[k]);
-        if ((a > (double)st->abs_tol) && (rr > (double)st->rel_tol)) {
-            atomicOr(s_notconv, 1);
-        } else if (!st->conv[k]) {
-            st->abs_res[k] = a;
-            st->rel_res[k] = rr;
-            st->conv[k] = 1;
-        }
-    }
-}
-
 __global__ __launch_bounds__(256) void k_finalize(int op, SolverState *__restrict__ st,
                                                   const double *__restrict__ partials,
                                                   int nparts, int nv) {
-    if (st->done)
-        return;
-    const int nd = (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) ? 3 : 1;
-    const int C = nd * nv;
-    const int S = 256 / C; // >= 1 since nd*nv <= 3*64 < 256
-    __shared__ double red[256];
-    __shared__ double tot[3 * kMaxNV];
-    __shared__ int s_notconv;
-    const int tid = threadIdx.x;
-    if (tid == 0)
-        s_notconv = 0;
-    if (tid < C * S) {
-        const int c = tid % C, sub = tid / C;
-        double acc = 0.0;
-        for (int g = sub; g < nparts; g += S)
-            acc = acc + partials[(size_t)g * C + c];
-        red[sub * C + c] = acc;
-    }
-    __syncthreads();
-    if (tid < C) {
-        double acc = red[tid];
-        for (int sub = 1; sub < S; ++sub)
-            acc = acc + red[sub * C + tid];
-        tot[tid] = acc;
-    }
-    __syncthreads();
-    const int k = tid;
-    const bool act = k < nv;
-    if (op == FIN_INIT) {
-        if (act) {
-            st->rho0[k] = tot[k];
-            if (st->hist && st->hist_len > 0)
-                st->hist[k] = sqrt(tot[k]);
-        }
-        __syncthreads();
-        if (st->conv_check)
-            conv_check(st, st->rho0, st->rho0, k, nv, &s_notconv);
-        __syncthreads();
-        if (!st->conv_check) {
-            if (act)
-                st->rho[k] = st->rho0[k];
-            return;
-        }
-        if (!s_notconv) {
-            if (tid == 0) {
-                st->conv_all = 1;
-                st->done = 1;
-            }
-            return;
-        }
-        if (act) {
-            st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
-            st->rho[k] = st->rho0[k];
-        }
-        return;
-    }
-    if (op == FIN_ALPHA) {
-        if (tid == 0)
-            st->iters += 1;
-        if (act) {
-            st->scal1[k] = tot[k];
-            const double al = st->rho[k] / tot[k]; // xdivy_z(rho, scal1, alpha)
-            st->alpha[k] = al;
-            st->nalpha[k] = al * -1.0;
-        }
-        return;
-    }
-    if (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) {
-        if (act) {
-            const double ph = tot[k], ps = tot[nv + k], th = tot[2 * nv + k];
-            st->phi[k] = ph;
-            st->psi[k] = ps;
-            st->theta[k] = th;
-            const double om = ph / ps;      // xdivy_z(phi, psi, omega)
-            st->omega[k] = om;
-            st->nomega[k] = om * -1.0;
-            const double wr = om * ph;      // ax_y(omega, phi, res)
-            st->res[k] = th + -1.0 * wr;    // axpby(a1, theta, a_1, res)
-            st->ac[k] = st->iconv[k] * st->alpha[k];
-            st->oc[k] = st->iconv[k] * om;
-            const int it = st->iters;
-            if (st->hist && it < st->hist_len)
-                st->hist[(size_t)it * nv + k] = sqrt(st->res[k]);
-        }
-        __syncthreads();
-        if (!st->conv_check)
-            return;
-        conv_check(st, st->res, st->rho0, k, nv, &s_notconv);
-        __syncthreads();
-        if (!s_notconv) {
-            if (tid == 0)
-                st->conv_all = 1;
-            return;
-        }
-        if (act) {
-            st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
-            if (op == FIN_OMEGA_MERGED) {
-                st->ac[k] = st->iconv[k] * st->alpha[k];
-                st->oc[k] = st->iconv[k] * st->omega[k];
-            }
-        }
-        return;
-    }
-    if (op == FIN_BETA) {
-        if (st->conv_all) {
-            if (tid == 0)
-                st->done = 1;
-            return;
-        }
-        if (act) {
-            const double rn = tot[k];
-            const double s2 = rn / st->rho[k];          // xdivy_z(scal1, rho, scal2)
-            const double s3 = st->alpha[k] / st->omega[k]; // xdivy_z(alpha, omega, scal3)
-            const double be = s2 * s3;                  // ax_y(scal2, scal3, beta)
-            st->beta[k] = be;
-            st->rho[k] = rn;                            // copy(scal1, rho)
-            const double ob = st->omega[k] * be;        // ax_y(omega, beta, scal1)
-            st->pcoef[k] = ob * -1.0;                   // scal(a_1, scal1)
-        }
-        if (tid == 0 && st->iters >= st->max_iters)
-            st->done = 1;
-        return;
-    }
+    finalize_body(op, st, partials, nparts, nv, (op == FIN_OMEGA || op == FIN_OMEGA_MERGED ? 3 : 1) * nv);
 }
 
 // ---- smoother / coarse / conversion kernels ----

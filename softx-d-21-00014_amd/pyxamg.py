@@ -97,6 +97,27 @@ def _load():
         "xh_matrix_create_compressed": [i64, i64, i64, i64, vp, vp, C.c_int, vp, vp, i64, i64,
                                         vp, C.c_int, P(vp)],
         "xh_matrix_gather": [vp, C.c_int, vp, vp],
+        "xh_batch_create": [C.c_int, vp, vp, vp, vp, vp, vp, P(Params), P(vp)],
+        "xh_batch_set_rhs": [vp, C.c_int, vp],
+        "xh_batch_solve": [vp],
+        "xh_batch_get_x": [vp, C.c_int, vp],
+        "xh_batch_stats": [vp, C.c_int, P(i32), vp, vp, vp, P(f64)],
+        "xh_batch_autotune": [vp, C.c_int, vp],
+        "xh_batch_set_tiles": [vp, vp],
+        "xh_batch_destroy": [vp],
+        "xh_unet_create": [C.c_int, vp, u64, P(vp)],
+        "xh_unet_destroy": [vp],
+        "xh_unet_nparams": [vp, P(i64)],
+        "xh_unet_get_params": [vp, vp],
+        "xh_unet_set_params": [vp, vp],
+        "xh_unet_get_output_scale": [vp, P(C.c_float)],
+        "xh_unet_set_output_scale": [vp, C.c_float],
+        "xh_unet_loss_grad": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, P(C.c_float), vp,
+                              vp],
+        "xh_unet_train": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int,
+                          C.c_int, C.c_int, C.c_float, u64, vp],
+        "xh_unet_infer": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp],
+        "xh_unet_mfma_selftest": [vp, vp, vp],
         "xh_hier_create": [C.c_int, P(vp)],
         "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
                               vp, vp, f64, f64],
@@ -384,6 +405,135 @@ class Solver:
         if getattr(self, "s", None) and _hip is not None:
             _hip.xh_solver_destroy(self.s)
             self.s = None
+
+
+class Batch:
+    """Batched independent systems (include/xamg_hip.h: xh_batch_*).  systems: list of
+    (row, col, val); nvs: RHS count per system; lmax: largest eigenvalue bound of
+    D^-1/2 A D^-1/2 per system (Chebyshev)."""
+
+    def __init__(self, systems, nvs, params, lmax=None):
+        self.ns = len(systems)
+        self.keep = [(np.ascontiguousarray(r, np.uint32), np.ascontiguousarray(c, np.uint32),
+                      np.ascontiguousarray(v, np.float64)) for r, c, v in systems]
+        self.n = np.array([len(r) - 1 for r, _, _ in self.keep], np.int64)
+        self.nvs = np.ascontiguousarray(nvs, np.int32)
+        rows = (C.c_void_p * self.ns)(*[r.ctypes.data for r, _, _ in self.keep])
+        cols = (C.c_void_p * self.ns)(*[c.ctypes.data for _, c, _ in self.keep])
+        vals = (C.c_void_p * self.ns)(*[v.ctypes.data for _, _, v in self.keep])
+        self.lmax = None if lmax is None else np.ascontiguousarray(lmax, np.float64)
+        self.h = C.c_void_p()
+        check(hip().xh_batch_create(self.ns, _ptr(self.n), _ptr(self.nvs), rows, cols, vals,
+                                    None if self.lmax is None else _ptr(self.lmax),
+                                    C.byref(params), C.byref(self.h)), "xh_batch_create")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _hip is not None:
+            _hip.xh_batch_destroy(self.h)
+            self.h = None
+
+    def set_rhs(self, s, b):
+        b = np.ascontiguousarray(b, np.float64)
+        assert b.size == self.n[s] * self.nvs[s]
+        check(hip().xh_batch_set_rhs(self.h, s, _ptr(b)), "xh_batch_set_rhs")
+
+    def solve(self):
+        check(hip().xh_batch_solve(self.h), "xh_batch_solve")
+
+    def x(self, s):
+        out = np.zeros(int(self.n[s] * self.nvs[s]))
+        check(hip().xh_batch_get_x(self.h, s, _ptr(out)), "xh_batch_get_x")
+        return out
+
+    def stats(self, s):
+        it = C.c_int32()
+        nv = int(self.nvs[s])
+        conv, a, r = np.zeros(nv, np.int32), np.zeros(nv), np.zeros(nv)
+        lat = C.c_double()
+        check(hip().xh_batch_stats(self.h, s, C.byref(it), _ptr(conv), _ptr(a), _ptr(r),
+                                   C.byref(lat)), "xh_batch_stats")
+        return dict(iters=it.value, conv=conv, abs_res=a, rel_res=r, latency_ms=lat.value)
+
+    def autotune(self, reps=5):
+        t = np.zeros(5, np.int32)
+        check(hip().xh_batch_autotune(self.h, reps, _ptr(t)), "xh_batch_autotune")
+        return t
+
+    def set_tiles(self, tiles):
+        t = np.ascontiguousarray(tiles, np.int32)
+        check(hip().xh_batch_set_tiles(self.h, _ptr(t)), "xh_batch_set_tiles")
+
+
+class UNet:
+    """Learned initial guess (include/xamg_unet.h): a 3D U-Net on the GPU."""
+
+    def __init__(self, channels, seed=0):
+        self.channels = np.ascontiguousarray(channels, np.int32)
+        self.h = C.c_void_p()
+        check(hip().xh_unet_create(len(self.channels), _ptr(self.channels), seed,
+                                   C.byref(self.h)), "xh_unet_create")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _hip is not None:
+            _hip.xh_unet_destroy(self.h)
+            self.h = None
+
+    def nparams(self):
+        n = C.c_int64()
+        check(hip().xh_unet_nparams(self.h, C.byref(n)), "xh_unet_nparams")
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.nparams(), np.float32)
+        check(hip().xh_unet_get_params(self.h, _ptr(p)), "xh_unet_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams()
+        check(hip().xh_unet_set_params(self.h, _ptr(p)), "xh_unet_set_params")
+
+    @property
+    def output_scale(self):
+        o = C.c_float()
+        check(hip().xh_unet_get_output_scale(self.h, C.byref(o)), "xh_unet_get_output_scale")
+        return o.value
+
+    @output_scale.setter
+    def output_scale(self, o):
+        check(hip().xh_unet_set_output_scale(self.h, float(o)), "xh_unet_set_output_scale")
+
+    def loss_grad(self, x_in, tgt):
+        """x_in, tgt: float32 [n, d, h, w]; returns (loss, flat gradient, y)"""
+        x_in = np.ascontiguousarray(x_in, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        n, d, h, w = x_in.shape
+        loss = C.c_float()
+        g = np.zeros(self.nparams(), np.float32)
+        y = np.zeros(x_in.size, np.float32)
+        check(hip().xh_unet_loss_grad(self.h, n, d, h, w, _ptr(x_in), _ptr(tgt), C.byref(loss),
+                                      _ptr(g), _ptr(y)), "xh_unet_loss_grad")
+        return loss.value, g, y.reshape(x_in.shape)
+
+    def train(self, nx, ny, nz, pairs, nv, crop, batch, steps, lr=1e-3, seed=0):
+        """pairs: list of (b, x) DeviceArray multivectors (nx*ny*nz rows, nv columns)"""
+        bs = (C.c_void_p * len(pairs))(*[b.ptr for b, _ in pairs])
+        xs = (C.c_void_p * len(pairs))(*[x.ptr for _, x in pairs])
+        hist = np.zeros(max(steps, 1), np.float32)
+        check(hip().xh_unet_train(self.h, nx, ny, nz, len(pairs), bs, xs, nv, crop, batch, steps,
+                                  lr, seed, _ptr(hist)), "xh_unet_train")
+        return hist[:steps]
+
+    def infer(self, nx, ny, nz, nv, db, dx0):
+        check(hip().xh_unet_infer(self.h, nx, ny, nz, nv, db.ptr, dx0.ptr), "xh_unet_infer")
+
+
+def mfma_selftest(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    d = np.zeros((16, 16), np.float32)
+    check(hip().xh_unet_mfma_selftest(_ptr(a), _ptr(b), _ptr(d)), "xh_unet_mfma_selftest")
+    return d
 
 
 class DeviceArray:

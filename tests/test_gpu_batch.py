@@ -1,0 +1,96 @@
+"""Batched independent systems on the GPU: every system of the batch against the oracle's
+single-system solve of that system (iteration count within 1, per-RHS relative solution
+difference <= 1e-6), graph replay against eager launches, autotuned work-item sizes, and
+the per-system statistics."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def systems(X, sizes, sigma=1.0, seed=10):
+    out = []
+    for i, N in enumerate(sizes):
+        row, col, val, _ = X.varcoef7(N, sigma=sigma, seed=seed + i)
+        lv = X.Amg(row, col, val, max_levels=1).export()
+        out.append((row, col, val, lv))
+    return out
+
+
+def params(X, method="PBiCGStab", precond="Chebyshev", order=8, graph=1):
+    return X.default_params(method=method, precond=precond, poly_order=order, max_iters=500,
+                            rel_tol=1e-8, graph=graph)
+
+
+def run_batch(X, sy, nvs, prm, seed=5):
+    B = X.Batch([(r, c, v) for r, c, v, _ in sy], nvs, prm,
+                lmax=[lv[0]["eig"][1] for _, _, _, lv in sy])
+    bs = []
+    for s, (r, _, _, _) in enumerate(sy):
+        b = X.rhs_uniform(seed + s, len(r) - 1, nvs[s])
+        B.set_rhs(s, b)
+        bs.append(b)
+    B.solve()
+    return B, bs
+
+
+def rel_diff(a, b, nv):
+    a, b = a.reshape(-1, nv), b.reshape(-1, nv)
+    return (np.abs(a - b).max(0) / np.abs(b).max(0)).max()
+
+
+@pytest.mark.parametrize("method,precond", [("PBiCGStab", "Chebyshev"),
+                                            ("MergedPBiCGStab", "Identity"),
+                                            ("BiCGStab", None)])
+def test_batch_matches_single_system_oracle(gpu, O, method, precond):
+    X = gpu
+    sizes = (6, 9, 12, 7, 10, 14)
+    nvs = (1, 2, 4, 8, 16, 4)
+    sy = systems(X, sizes)
+    prm = params(X, method, precond)
+    B, bs = run_batch(X, sy, nvs, prm)
+    for s, (r, c, v, lv) in enumerate(sy):
+        st = B.stats(s)
+        ref = O.solve(O.Hier(lv), O.params_from(prm), nvs[s], bs[s])
+        assert st["conv"].all() and ref["conv"].all()
+        assert abs(st["iters"] - ref["iters"]) <= 1, (s, st["iters"], ref["iters"])
+        assert rel_diff(B.x(s), ref["x"], nvs[s]) <= 1e-6
+        assert st["latency_ms"] > 0
+
+
+def test_batch_graph_eager_and_retiling(gpu):
+    X = gpu
+    sizes = (8, 13, 5, 11)
+    nvs = (4, 1, 16, 8)
+    sy = systems(X, sizes, seed=30)
+    xs = []
+    for graph in (0, 1):
+        B, _ = run_batch(X, sy, nvs, params(X, graph=graph))
+        xs.append([B.x(s) for s in range(len(sizes))])
+        its = [B.stats(s)["iters"] for s in range(len(sizes))]
+    for a, b in zip(*xs):
+        assert np.array_equal(a, b)
+    # a second solve on the same batch (graph replay) gives the same result
+    B.solve()
+    for s in range(len(sizes)):
+        assert np.array_equal(B.x(s), xs[1][s])
+    tiles = B.autotune(reps=3)
+    assert all(t in (1, 2, 4, 8, 16, 32) for t in tiles)
+    B.solve()
+    for s in range(len(sizes)):
+        # other tiles change only the order of the dot partial sums
+        assert abs(B.stats(s)["iters"] - its[s]) <= 1
+        assert rel_diff(B.x(s), xs[1][s], nvs[s]) <= 1e-6
+    B.set_tiles([1, 1, 1, 1, 1])
+    B.solve()
+    assert all(B.stats(s)["conv"].all() for s in range(len(sizes)))
+
+
+def test_batch_rejects_unsupported(gpu):
+    X = gpu
+    sy = systems(X, (5,))
+    r, c, v, lv = sy[0]
+    with pytest.raises(X.XamgError):  # nv = 3
+        X.Batch([(r, c, v)], [3], params(X), lmax=[lv[0]["eig"][1]])
+    with pytest.raises(X.XamgError):  # a multigrid preconditioner is not batched
+        X.Batch([(r, c, v)], [1], params(X, precond="MultiGrid"), lmax=[lv[0]["eig"][1]])

@@ -1,0 +1,102 @@
+"""Learned initial guess on the GPU: the MFMA operand layout, the network's forward pass,
+loss and every parameter gradient against the CPU oracle (oracle/unet_oracle.py with the
+GPU's bf16 rounding points), and training on solver-generated pairs with the solver run
+from the network's x0."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+import unet_oracle as UO  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_operand_layout(gpu):
+    X = gpu
+    rng = np.random.default_rng(0)
+    a = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
+    b = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    assert np.array_equal(X.mfma_selftest(a, b), a @ b)  # small integers: exact
+
+
+def layer_slices(channels):
+    shapes = []
+    for l in range(len(channels)):
+        shapes += [(1 if l == 0 else channels[l - 1], channels[l]), (channels[l], channels[l])]
+    for l in range(len(channels) - 1):
+        shapes += [(channels[l + 1], channels[l]), (channels[l], channels[l])]
+    out, o = [], 0
+    for a, b in shapes:
+        out.append(slice(o, o + 27 * a * b))
+        o += 27 * a * b
+        out.append(slice(o, o + b))
+        o += b
+    out.append(slice(o, o + channels[0] + 1))
+    return out
+
+
+@pytest.mark.parametrize("channels,size,batch", [((16, 32, 48), 8, 2),
+                                                 ((16, 32, 48, 64), 8, 1),
+                                                 ((16,), 6, 3)])
+def test_loss_and_gradients_match_oracle(gpu, channels, size, batch):
+    X = gpu
+    net = X.UNet(channels, seed=7)
+    p = net.get_params()
+    # non-zero biases so that every bias path is exercised
+    rng = np.random.default_rng(1)
+    for sl in layer_slices(channels)[1::2]:
+        p[sl] = rng.standard_normal(sl.stop - sl.start).astype(np.float32) * 0.1
+    net.set_params(p)
+    x = rng.standard_normal((batch, size, size, size)).astype(np.float32)
+    t = rng.standard_normal((batch, size, size, size)).astype(np.float32)
+    loss, g, y = net.loss_grad(x, t)
+    ref = UO.UNet(channels, p.astype(np.float64))
+    rloss, rg, ry = ref.loss_grad(x.astype(np.float64), t.astype(np.float64), use_bf16=True)
+    assert abs(loss - rloss) <= 1e-3 * rloss
+    assert np.linalg.norm(y - ry) <= 1e-2 * np.linalg.norm(ry)
+    for sl in layer_slices(channels):
+        gg, rr = g[sl], rg[sl]
+        assert np.linalg.norm(gg - rr) <= 3e-2 * np.linalg.norm(rr) + 1e-9, sl
+
+
+def test_training_and_solver_from_learned_x0(gpu):
+    X = gpu
+    N, nv = 32, 8
+    row, col, val, _ = X.varcoef7(N, sigma=1.0, seed=3)
+    levels = X.Amg(row, col, val, agg_num_levels=2, num_paths=2, coarse_size=500).export()
+    prm = X.default_params(method="MergedPBiCGStab", smoother="Chebyshev", poly_order=4,
+                           rel_tol=1e-8, max_iters=100)
+    S = X.Solver(X.Hier(levels), prm, nv)
+    pairs = []
+    for q in range(6):
+        b = X.rhs_bumps(100 + q, N, N, N, nv)
+        db, dx = X.DeviceArray.from_host(b), X.DeviceArray(N ** 3 * nv)
+        S.solve_device(db, dx)
+        assert S.stats()[1].all()
+        pairs.append((db, dx))
+    net = X.UNet((16, 32, 48, 64), seed=1)
+    hist = net.train(N, N, N, pairs, nv, crop=16, batch=8, steps=200, lr=2e-3, seed=5)
+    assert np.isfinite(hist).all()
+    assert hist[-20:].mean() < 0.5 * hist[:20].mean()
+    # held-out right-hand sides
+    A = X.Matrix(row, col, val)
+    b = X.rhs_bumps(999, N, N, N, nv)
+    db = X.DeviceArray.from_host(b)
+    dx0 = X.DeviceArray(N ** 3 * nv)
+    net.infer(N, N, N, nv, db, dx0)
+    ax = X.DeviceArray(N ** 3 * nv)
+    A.spmv(nv, dx0, ax)
+    r0 = np.linalg.norm((b - ax.to_host()).reshape(-1, nv), axis=0)
+    bn = np.linalg.norm(b.reshape(-1, nv), axis=0)
+    assert (r0 < bn).all()  # x0 is closer than 0
+    it = {}
+    for name, start in (("zero", None), ("learned", dx0.to_host())):
+        dx = X.DeviceArray.from_host(start) if start is not None else X.DeviceArray(N ** 3 * nv)
+        S.solve_device(db, dx)
+        st = S.stats()
+        assert st[1].all()
+        it[name] = st[0]
+    assert it["learned"] <= it["zero"], it
