@@ -83,6 +83,11 @@ int xamg_q1fem27(int64_t nx, int64_t ny, int64_t nz, const double *kcell, uint32
  * [min_width, max_width], amplitudes standard normal), cut off below exp(-8) */
 int xamg_rhs_bumps(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
                    int max_bumps, double min_width, double max_width, double *b);
+/* the bumps of xamg_rhs_bumps: count[k] per RHS, params[(k*max_bumps + q)*5 + {cx, cy, cz,
+ * w, a}] (for evaluation on the device, xh_eval_bumps) */
+int xamg_bump_params(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
+                     int max_bumps, double min_width, double max_width, int32_t *count,
+                     double *params);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,15 @@ int xh_matrix_create_compressed(int64_t y_rows, int64_t nrows, int64_t ncols, in
  * rows, x_c has ncols rows, both nv-interleaved in the matrix precision */
 int xh_matrix_gather(xh_matrix A, int nv, const void *dx_full, void *dx_c);
 
+/* Smooth right-hand sides on the device (no reference counterpart; the learned-initial-
+ * guess workload): db[p*nv + k] = sum over q < count[k] of
+ * a exp(-|x_p - c|^2 / (2 w^2)) with (cx, cy, cz, w, a) = params[(k*max_bumps + q)*5 ..],
+ * x_p the cell centre (x + 0.5, y + 0.5, z + 0.5), cells outside the box of half-width
+ * ceil(4w) around c skipped (the parameters of xamg_bump_params, the definition of
+ * xamg_rhs_bumps) */
+int xh_eval_bumps(int nx, int ny, int nz, int nv, int max_bumps, const int32_t *count,
+                  const double *params, double *db);
+
 /* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
 int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);
 

@@ -191,6 +191,10 @@ void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int
 
 // smoother / transfer helpers
 // `done` (nullable): device flag; when set the launch is a no-op
+// bump right-hand sides on the device (count[nv], P[nv][maxb][5] device arrays)
+void eval_bumps(int nx, int ny, int nz, int nv, int maxb, const int *count, const double *P,
+                double *b, hipStream_t s);
+
 // error message for xh_last_error(); returns `code`
 int report_error(int code, const std::string &msg);
 // selects device 0 on first use

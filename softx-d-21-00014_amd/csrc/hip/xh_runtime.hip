@@ -2045,3 +2045,32 @@ extern "C" int xh_batch_destroy(xh_batch B) {
         free_batch(B);
     return 0;
 }
+
+extern "C" int xh_eval_bumps(int nx, int ny, int nz, int nv, int max_bumps,
+                             const int32_t *count, const double *params, double *db) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (nx < 1 || ny < 1 || nz < 1 || nv < 1 || max_bumps < 1 || !count || !params || !db)
+        return fail(XH_EINVAL, "xh_eval_bumps: bad arguments");
+    for (int k = 0; k < nv; ++k)
+        if (count[k] < 0 || count[k] > max_bumps)
+            return fail(XH_EINVAL, "xh_eval_bumps: count out of range");
+    int *dc = nullptr;
+    double *dp = nullptr;
+    XH_HIP(hipMalloc((void **)&dc, nv * sizeof(int)));
+    hipError_t e = hipMalloc((void **)&dp, (size_t)nv * max_bumps * 5 * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(dc, count, nv * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dp, params, (size_t)nv * max_bumps * 5 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        eval_bumps(nx, ny, nz, nv, max_bumps, dc, dp, db, g_stream);
+        e = hipStreamSynchronize(g_stream);
+    }
+    (void)hipFree(dc);
+    (void)hipFree(dp);
+    if (e != hipSuccess)
+        return fail(XH_EHIP, std::string("xh_eval_bumps: ") + hipGetErrorString(e));
+    return 0;
+}

@@ -352,7 +352,7 @@ __global__ void k_head_bwd(int64_t M, int C, const bf16_t *__restrict__ h, const
     }
 }
 
-// loss partials: 0.5 (y - t)^2 summed per chunk; gy = (y - t) * inv_m
+// loss partials: 0.5 (y - t)^2 * inv_m summed per chunk; gy = (y - t) * inv_m
 __global__ void k_loss(int64_t M, const float *__restrict__ y, const float *__restrict__ tgt,
                        float inv_m, float *__restrict__ gy, int64_t chunk, float *__restrict__ part) {
     __shared__ float red[256];
@@ -361,7 +361,7 @@ __global__ void k_loss(int64_t M, const float *__restrict__ y, const float *__re
     for (int64_t m = c0 + threadIdx.x; m < c1; m += blockDim.x) {
         const float d = y[m] - tgt[m];
         gy[m] = d * inv_m;
-        s = s + 0.5f * d * d;
+        s = s + 0.5f * d * d * inv_m;
     }
     red[threadIdx.x] = s;
     __syncthreads();

@@ -344,6 +344,34 @@ __global__ __launch_bounds__(kBlock) void k_gather_cols(int64_t count, int nv,
     }
 }
 
+// b[p][k] = sum over the bumps of RHS k of a exp(-|x_p - c|^2 / (2 w^2)), cells within the
+// box of half-width ceil(4w) around c only (cell centres at integer + 0.5)
+__global__ __launch_bounds__(kBlock) void k_bumps(int nx, int ny, int nz, int nv, int maxb,
+                                                  const int *__restrict__ count,
+                                                  const double *__restrict__ P,
+                                                  double *__restrict__ b) {
+    const int64_t n = (int64_t)nx * ny * nz * nv;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int k = (int)(e % nv);
+        const int64_t p = e / nv;
+        const int x = (int)(p % nx), y = (int)((p / nx) % ny), z = (int)(p / ((int64_t)nx * ny));
+        double s = 0.0;
+        for (int q = 0; q < count[k]; ++q) {
+            const double *Q = P + ((size_t)k * maxb + q) * 5;
+            const double w = Q[3];
+            const int64_t R = (int64_t)ceil(4.0 * w);
+            if (x < (int64_t)Q[0] - R || x > (int64_t)Q[0] + R || y < (int64_t)Q[1] - R ||
+                y > (int64_t)Q[1] + R || z < (int64_t)Q[2] - R || z > (int64_t)Q[2] + R)
+                continue;
+            const double dx = (double)x + 0.5 - Q[0], dy = (double)y + 0.5 - Q[1],
+                         dz = (double)z + 0.5 - Q[2];
+            s += Q[4] * exp(-(dx * dx + dy * dy + dz * dz) * (1.0 / (2.0 * w * w)));
+        }
+        b[e] = s;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill0(int64_t count, T *__restrict__ b,
                                                   const int *__restrict__ done) {
@@ -525,6 +553,13 @@ void gather_cols(int64_t nc, int nv, int prec, const uint32_t *col_ind, int64_t 
     else
         hipLaunchKernelGGL(k_gather_cols<float>, dim3(grid_for(count)), dim3(kBlock), 0, s,
                            count, nv, col_ind, off, (const float *)x_full, (float *)x_c);
+}
+
+void eval_bumps(int nx, int ny, int nz, int nv, int maxb, const int *count, const double *P,
+                double *b, hipStream_t s) {
+    const int64_t n = (int64_t)nx * ny * nz * nv;
+    hipLaunchKernelGGL(k_bumps, dim3(grid_for(n)), dim3(kBlock), 0, s, nx, ny, nz, nv, maxb,
+                       count, P, b);
 }
 
 void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s) {

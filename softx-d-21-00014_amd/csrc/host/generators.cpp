@@ -160,27 +160,48 @@ int xamg_q1fem27(int64_t nx, int64_t ny, int64_t nz, const double *kcell, uint32
     return 0;
 }
 
-int xamg_rhs_bumps(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
-                   int max_bumps, double min_width, double max_width, double *b) {
-    // RHS k (of nv) is the sum of m in [min_bumps, max_bumps] Gaussian bumps
-    // a * exp(-|x - c|^2 / (2 w^2)) in cell units, c uniform over the grid, w uniform in
-    // [min_width, max_width], a standard normal; b[p*nv + k]
+int xamg_bump_params(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
+                     int max_bumps, double min_width, double max_width, int32_t *count,
+                     double *params) {
+    // RHS k: count[k] bumps, bump q described by params[(k*max_bumps + q)*5 + {cx, cy, cz,
+    // w, a}] (centre in cell units, width, amplitude)
     if (nx < 1 || ny < 1 || nz < 1 || nv < 1 || min_bumps < 1 || max_bumps < min_bumps ||
-        min_width <= 0 || max_width < min_width || !b)
+        min_width <= 0 || max_width < min_width || !count || !params)
         return -1;
-    std::memset(b, 0, sizeof(double) * nx * ny * nz * nv);
     for (int k = 0; k < nv; ++k) {
         const uint64_t s = seed * 1000003ull + (uint64_t)k;
         const int m = min_bumps + (int)(unif(s, 0) * (max_bumps - min_bumps + 1));
-        for (int q = 0; q < m; ++q) {
+        count[k] = m;
+        for (int q = 0; q < max_bumps; ++q) {
+            double *P = params + ((size_t)k * max_bumps + q) * 5;
             const uint64_t o = 8 + 8 * (uint64_t)q;
-            const double ccx = unif(s, o) * nx, ccy = unif(s, o + 1) * ny,
-                         ccz = unif(s, o + 2) * nz;
-            const double w = min_width + unif(s, o + 3) * (max_width - min_width);
-            const double a = normal(s, o + 4);
+            P[0] = unif(s, o) * nx;
+            P[1] = unif(s, o + 1) * ny;
+            P[2] = unif(s, o + 2) * nz;
+            P[3] = min_width + unif(s, o + 3) * (max_width - min_width);
+            P[4] = normal(s, o + 4);
+        }
+    }
+    return 0;
+}
+
+int xamg_rhs_bumps(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, int min_bumps,
+                   int max_bumps, double min_width, double max_width, double *b) {
+    // sum over the bumps of xamg_bump_params; a bump is cut off outside the box of
+    // half-width ceil(4w) cells around its centre (it has fallen below exp(-8) there)
+    if (!b)
+        return -1;
+    std::vector<int32_t> count(nv > 0 ? nv : 1);
+    std::vector<double> P((size_t)(nv > 0 ? nv : 1) * (max_bumps > 0 ? max_bumps : 1) * 5);
+    if (xamg_bump_params(seed, nx, ny, nz, nv, min_bumps, max_bumps, min_width, max_width,
+                         count.data(), P.data()))
+        return -1;
+    std::memset(b, 0, sizeof(double) * nx * ny * nz * nv);
+    for (int k = 0; k < nv; ++k)
+        for (int q = 0; q < count[k]; ++q) {
+            const double *Q = &P[((size_t)k * max_bumps + q) * 5];
+            const double ccx = Q[0], ccy = Q[1], ccz = Q[2], w = Q[3], a = Q[4];
             const double inv = 1.0 / (2.0 * w * w);
-            // the bump is cut off outside the box of half-width ceil(4w) cells around its
-            // centre (it has fallen below exp(-8) there)
             const int64_t R = (int64_t)std::ceil(4.0 * w);
             auto lo = [](double c0, int64_t r) { return std::max<int64_t>(0, (int64_t)c0 - r); };
             auto hi = [](double c0, int64_t r, int64_t m) {
@@ -196,7 +217,6 @@ int xamg_rhs_bumps(uint64_t seed, int64_t nx, int64_t ny, int64_t nz, int nv, in
                         b[p * nv + k] += a * std::exp(-(dx * dx + dy * dy + dz * dz) * inv);
                     }
         }
-    }
     return 0;
 }
 

@@ -118,6 +118,7 @@ def _load():
                           C.c_int, C.c_int, C.c_float, u64, vp],
         "xh_unet_infer": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp],
         "xh_unet_mfma_selftest": [vp, vp, vp],
+        "xh_eval_bumps": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp],
         "xh_hier_create": [C.c_int, P(vp)],
         "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
                               vp, vp, f64, f64],
@@ -148,6 +149,7 @@ def _load():
         "xamg_varcoef7": [i64, i64, i64, vp, vp, vp, vp],
         "xamg_q1fem27": [i64, i64, i64, vp, vp, vp, vp],
         "xamg_rhs_bumps": [u64, i64, i64, i64, C.c_int, C.c_int, C.c_int, f64, f64, vp],
+        "xamg_bump_params": [u64, i64, i64, i64, C.c_int, C.c_int, C.c_int, f64, f64, vp, vp],
         "xamg_rhs_uniform": [u64, i64, vp],
     }
     for name, args in hsig.items():
@@ -238,6 +240,19 @@ def rhs_bumps(seed, nx, ny, nz, nv, min_bumps=8, max_bumps=32, min_width=2.0, ma
     check(host().xamg_rhs_bumps(seed, nx, ny, nz, nv, min_bumps, max_bumps, min_width,
                                 max_width, _ptr(b)), "xamg_rhs_bumps")
     return b
+
+
+def rhs_bumps_device(seed, nx, ny, nz, nv, min_bumps=8, max_bumps=32, min_width=2.0,
+                     max_width=12.0, out=None):
+    """the right-hand sides of rhs_bumps, evaluated on the GPU into a DeviceArray"""
+    count = np.zeros(nv, np.int32)
+    P = np.zeros(nv * max_bumps * 5)
+    check(host().xamg_bump_params(seed, nx, ny, nz, nv, min_bumps, max_bumps, min_width,
+                                  max_width, _ptr(count), _ptr(P)), "xamg_bump_params")
+    out = DeviceArray(nx * ny * nz * nv) if out is None else out
+    check(hip().xh_eval_bumps(nx, ny, nz, nv, max_bumps, _ptr(count), _ptr(P), out.ptr),
+          "xh_eval_bumps")
+    return out
 
 
 def csr_fold(row, col, ncols=None):

@@ -44,3 +44,10 @@ def test_varcoef7_chebyshev8_without_amg(gpu, O):
     assert g["conv"].all() and r["conv"].all()
     assert abs(g["iters"] - r["iters"]) <= 1
     assert rel_diff(g["x"], r["x"], nv) <= 1e-6
+
+
+def test_device_bump_rhs_matches_host_generator(gpu):
+    X = gpu
+    b_host = X.rhs_bumps(17, 24, 20, 16, 4)
+    b_dev = X.rhs_bumps_device(17, 24, 20, 16, 4).to_host()
+    assert np.allclose(b_dev, b_host, rtol=1e-12, atol=1e-14 * np.abs(b_host).max())
