@@ -92,10 +92,15 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
     const int r = lane & 15, kg = lane >> 4;
     const int64_t m = m0 + r;
     const bool mok = m < M;
+    // the lane's voxel, decoded once
+    const int vx = (int)(m % g.w), vy = (int)((m / g.w) % g.h),
+              vz = (int)((m / ((int64_t)g.w * g.h)) % g.d);
     f32x4 acc[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c)
         acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // tap and channel of the lane's first k, advanced by 32 per step (cin % 8 == 0)
+    int tt = (8 * kg) / cin, cc = 8 * kg - tt * cin;
     for (int kb = 0; kb < kpad; kb += 32) {
         const int k0 = kb + 8 * kg;
         bf16x8 a;
@@ -103,12 +108,19 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
             // 8 consecutive channels of one tap
             uint4 q = make_uint4(0, 0, 0, 0);
             if (mok && k0 < K) {
-                const int t = k0 / cin, ci = k0 - t * cin;
-                const int64_t nb = neigh(g, m, t);
-                if (nb >= 0)
-                    q = *reinterpret_cast<const uint4 *>(in + nb * cin + ci);
+                const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+                const int zz = vz + dz, yy = vy + dy, xx = vx + dx;
+                if (zz >= 0 && zz < g.d && yy >= 0 && yy < g.h && xx >= 0 && xx < g.w) {
+                    const int64_t nb = m + ((int64_t)dz * g.h + dy) * g.w + dx;
+                    q = *reinterpret_cast<const uint4 *>(in + nb * cin + cc);
+                }
             }
             a = __builtin_bit_cast(bf16x8, q);
+            cc += 32;
+            while (cc >= cin) {
+                cc -= cin;
+                ++tt;
+            }
         } else {
             bf16_t e[8];
 #pragma unroll
