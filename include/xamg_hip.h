@@ -49,6 +49,9 @@ typedef struct xh_batch_s *xh_batch;
 #define XH_PBICGSTAB 7
 #define XH_MERGED_BICGSTAB 8
 #define XH_MERGED_PBICGSTAB 9
+#define XH_PCG 10
+#define XH_RBICGSTAB 11
+#define XH_MERGED_RBICGSTAB 12
 #define XH_NONE -1
 
 /* Runtime init: replaces XAMG::init's device part (src/init.h:54-76; the reference's

@@ -12,6 +12,7 @@ LIB = os.path.join(HERE, "build", "libxamg_oracle.so")
 
 IDENTITY, JACOBI, HSGS, CHEBYSHEV, DIRECT, MULTIGRID = 0, 1, 2, 3, 4, 5
 BICGSTAB, PBICGSTAB, MERGED_BICGSTAB, MERGED_PBICGSTAB = 6, 7, 8, 9
+PCG, RBICGSTAB, MERGED_RBICGSTAB = 10, 11, 12
 NONE = -1
 
 

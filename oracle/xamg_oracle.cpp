@@ -1310,10 +1310,10 @@ struct Krylov : Solver<double> {
 
 struct Scalars {
     Vec<double> rho, rho0, res, scal1, scal2, scal3, scal4, alpha, beta, omega, ac, oc, theta,
-        phi, psi;
+        phi, psi, delta, eta, temp;
     explicit Scalars(int nv) {
         for (Vec<double> *v : {&rho, &rho0, &res, &scal1, &scal2, &scal3, &scal4, &alpha, &beta,
-                               &omega, &ac, &oc, &theta, &phi, &psi})
+                               &omega, &ac, &oc, &theta, &phi, &psi, &delta, &eta, &temp})
             v->alloc(1, nv);
     }
 };
@@ -1538,6 +1538,154 @@ struct MergedBiCGStab : Krylov { /* merged_bicgstab.inl:44-208 */
     }
 };
 
+struct PCG : Krylov { /* pcg.inl:44-163 */
+    void solve(Vec<double> &x, Vec<double> &b, Vec<double> &conv) override {
+        const int nv = x.nv;
+        this->nv = nv;
+        reset();
+        auto &it = stats.iters;
+        Vec<double> iconv = conv;
+        Vec<double> a1 = const_vec<double>(nv, 1.0), a_1 = const_vec<double>(nv, -1.0);
+        Scalars S(nv);
+        Vec<double> r, z, p, Ap;
+        for (Vec<double> *q : {&r, &z, &p, &Ap})
+            q->alloc(x.n, nv);
+        get_residual(x, b, r, S.rho0);
+        if (converged(S.rho0, S.rho0, iconv))
+            return;
+        set_const(z, 0.0);
+        precond->solve(z, r);
+        copy(z, p);
+        do {
+            ++it;
+            Ax_y(*A, p, Ap);
+            dot(r, z, S.scal1);
+            dot(p, Ap, S.scal2);
+            xdivy_z(S.scal1, S.scal2, S.alpha);
+            ax_y(iconv, S.alpha, S.ac);
+            axpby(S.ac, p, a1, x);
+            scal(a_1, S.alpha);
+            axpby(S.alpha, Ap, a1, r);
+            set_const(z, 0.0);
+            precond->solve(z, r);
+            dot(r, z, S.scal3);
+            xdivy_z(S.scal3, S.scal1, S.beta);
+            axpby(a1, z, S.beta, p);
+            dot(r, r, S.res);
+            if (converged(S.res, S.rho0, iconv))
+                return;
+            if (g_err)
+                return;
+        } while ((int)it < prm.max_iters);
+    }
+};
+
+/* RBiCGStab (rbicgstab.inl:44-232) and MergedRBiCGStab (merged/merged_rbicgstab.inl:44-226):
+ * the same recurrences; the merged form fuses steps 12-13 (merged_rbicgstab_group1,
+ * blas_merged_rbicgstab.inl:33-118, the same roundings) and 23-25
+ * (merged_rbicgstab_group2, :120-196: v_ = z + beta v_ - beta omega s, a different
+ * association) and updates x after the convergence check with the updated iconv. */
+struct RBiCGStab : Krylov {
+    bool merged = false;
+    void solve(Vec<double> &x, Vec<double> &b, Vec<double> &conv) override {
+        const int nv = x.nv;
+        this->nv = nv;
+        reset();
+        auto &it = stats.iters;
+        Vec<double> iconv = conv;
+        Vec<double> a1 = const_vec<double>(nv, 1.0), a_1 = const_vec<double>(nv, -1.0);
+        Scalars S(nv);
+        Vec<double> r0, r, z, v_, v, s, t_, t;
+        for (Vec<double> *q : {&r0, &r, &z, &v_, &v, &s, &t_, &t})
+            q->alloc(x.n, nv);
+        get_residual(x, b, r0, S.rho0);
+        copy(r0, r);
+        if (converged(S.rho0, S.rho0, iconv))
+            return;
+        copy(S.rho0, S.rho);
+        set_const(z, 0.0);
+        precond->solve(z, r0);
+        copy(z, v_);
+        do {
+            ++it;
+            Ax_y(*A, v_, v);                          // 6
+            dot(v, r0, S.delta);                      // 7
+            set_const(s, 0.0);
+            precond->solve(s, v);                     // 8
+            xdivy_z(S.rho, S.delta, S.alpha);         // 9
+            scal(a_1, S.alpha);
+            axpby_z(a1, z, S.alpha, s, t_);           // 10
+            scal(a_1, S.alpha);
+            Ax_y(*A, t_, t);                          // 11
+            if (merged) {                             // 12-13, group1
+                const size_t N = r.sz();
+                for (int k = 0; k < nv; ++k)
+                    S.theta.d[k] = S.phi.d[k] = S.psi.d[k] = S.eta.d[k] = 0.0;
+                for (size_t i = 0; i < N; i += nv)
+                    for (int k = 0; k < nv; ++k) {
+                        r.d[i + k] -= S.alpha.d[k] * v.d[i + k];
+                        S.theta.d[k] += t.d[i + k] * r.d[i + k];
+                        S.phi.d[k] += t.d[i + k] * t.d[i + k];
+                        S.psi.d[k] += t.d[i + k] * r0.d[i + k];
+                        S.eta.d[k] += r.d[i + k] * r.d[i + k];
+                    }
+                for (Vec<double> *q : {&S.theta, &S.phi, &S.psi, &S.eta})
+                    q->zero = false;
+            } else {
+                scal(a_1, S.alpha);
+                axpby(S.alpha, v, a1, r);             // 12
+                scal(a_1, S.alpha);
+                dot(t, r, S.theta);                   // 13
+                dot(t, t, S.phi);
+                dot(t, r0, S.psi);
+                dot(r, r, S.eta);
+            }
+            set_const(z, 0.0);
+            precond->solve(z, t);                     // 14
+            xdivy_z(S.theta, S.phi, S.omega);         // 15
+            scal(a_1, S.omega);
+            axpby(S.omega, t, a1, r);                 // 16
+            scal(a_1, S.omega);
+            ax_y(S.omega, S.theta, S.res);            // 17
+            axpby(a1, S.eta, a_1, S.res);
+            ax_y(iconv, S.alpha, S.ac);
+            ax_y(iconv, S.omega, S.oc);
+            if (!merged)
+                axpbypcz(S.ac, v_, S.oc, t_, a1, x);  // 23
+            if (converged(S.res, S.rho0, iconv)) {
+                if (merged)
+                    axpbypcz(S.ac, v_, S.oc, t_, a1, x);
+                return;
+            }
+            ax_y(S.omega, S.psi, S.rho);              // 21
+            scal(a_1, S.rho);
+            ax_y(S.delta, S.omega, S.temp);           // 22
+            xdivy_z(S.rho, S.temp, S.beta);
+            if (!merged) {
+                scal(a_1, S.omega);
+                axpby(a1, t_, S.omega, z);            // 24
+                ax_y(S.omega, S.beta, S.delta);       // 25
+                scal(a_1, S.omega);
+                axpbypcz(a1, z, S.delta, s, S.beta, v_);
+            } else {                                  // group2 with the updated iconv
+                ax_y(iconv, S.alpha, S.ac);
+                ax_y(iconv, S.omega, S.oc);
+                const size_t N = x.sz();
+                for (size_t i = 0; i < N; i += nv)
+                    for (int k = 0; k < nv; ++k) {
+                        x.d[i + k] += S.ac.d[k] * v_.d[i + k] + S.oc.d[k] * t_.d[i + k];
+                        z.d[i + k] = t_.d[i + k] - S.omega.d[k] * z.d[i + k];
+                        v_.d[i + k] = z.d[i + k] + S.beta.d[k] * v_.d[i + k] -
+                                      S.beta.d[k] * S.omega.d[k] * s.d[i + k];
+                    }
+                x.zero = z.zero = v_.zero = false;
+            }
+            if (g_err)
+                return;
+        } while ((int)it < prm.max_iters);
+    }
+};
+
 static Params params_of(const xo_params *p) {
     Params P;
     P.solver = role_of(p->solver);
@@ -1585,7 +1733,8 @@ static int build_set(const xo_hier *H, const xo_params *prm, SolverSet &Z) {
     Z.A0 = make_csr<double>(&H->A[0]);
     Solver<double> *precond = nullptr;
     const int m = P.solver.method;
-    if (m == XO_PBICGSTAB || m == XO_MERGED_PBICGSTAB) {
+    if (m == XO_PBICGSTAB || m == XO_MERGED_PBICGSTAB || m == XO_PCG || m == XO_RBICGSTAB ||
+        m == XO_MERGED_RBICGSTAB) {
         Z.pre = make_precond(H, P, P.precond, &Z.A0, Z.mg);
         precond = Z.mg ? static_cast<Solver<double> *>(Z.mg.get()) : Z.pre.get();
         if (!precond)
@@ -1604,6 +1753,16 @@ static int build_set(const xo_hier *H, const xo_params *prm, SolverSet &Z) {
     case XO_MERGED_BICGSTAB:
         Z.top.reset(new MergedBiCGStab());
         break;
+    case XO_PCG:
+        Z.top.reset(new PCG());
+        break;
+    case XO_RBICGSTAB:
+    case XO_MERGED_RBICGSTAB: {
+        auto *rb = new RBiCGStab();
+        rb->merged = m == XO_MERGED_RBICGSTAB;
+        Z.top.reset(rb);
+        break;
+    }
     case XO_MULTIGRID:
         Z.mg.reset(new MultiGrid());
         Z.mg->setup(H, P);
@@ -1621,7 +1780,10 @@ static int build_set(const xo_hier *H, const xo_params *prm, SolverSet &Z) {
         k->A = &Z.A0;
         k->precond = precond;
         // MergedBiCGStab updates x only after r (group5), so b - A x is not r's residual
-        if (P.true_res_period > 0 && P.solver.method == XO_MERGED_BICGSTAB)
+        // periodic replacement is defined for the BiCGStab family of the benchmark only
+        if (P.true_res_period > 0 &&
+            !(P.solver.method == XO_BICGSTAB || P.solver.method == XO_PBICGSTAB ||
+              P.solver.method == XO_MERGED_PBICGSTAB))
             return -5;
         k->true_res_period = P.true_res_period;
     }

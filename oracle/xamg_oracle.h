@@ -34,6 +34,9 @@ enum {
     XO_PBICGSTAB = 7,
     XO_MERGED_BICGSTAB = 8,
     XO_MERGED_PBICGSTAB = 9,
+    XO_PCG = 10,
+    XO_RBICGSTAB = 11,
+    XO_MERGED_RBICGSTAB = 12,
     XO_NONE = -1
 };
 

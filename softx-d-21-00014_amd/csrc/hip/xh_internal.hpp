@@ -132,6 +132,8 @@ enum SpmvOp : int {
     OP_PROLONG32 = 7,// x64 = (double)(P32 xH32) + x64
     OP_OFFBLOCK = 8, // y = b - (off-block part of A) x, HSGS block partition
     OP_RESID0 = 9,   // y = b - A x, b and y f64, written both f64 and group1-fused
+    OP_RDOT4 = 10,   // t = A t_ ; r^ = r - alpha v (r: out2, v: b, alpha: rr) ; partials
+                     // (t, r^), (t, t), (t, r0), (r^, r^) (r0: o1)  (RBiCGStab steps 11-13)
 };
 
 struct SpmvArgs {
@@ -167,6 +169,16 @@ enum VecOp : int {
     V_GROUP5 = 6,     // x += ac*p + oc*s ; p = r + beta*p - (beta*omega)*v   (merged)
     V_DOT3 = 7,       // partials (t,s), (t,t), (s,s) of existing vectors
     V_DOT1 = 8,       // partial (a, b)
+    // PCG (pcg.inl:129-160)
+    V_CG_XR = 9,      // x = ac*p + x ; r = nalpha*Ap + r        (p: p_, Ap: t)
+    V_DOT2 = 10,      // partials (r, z), (r, r)                 (z: r0)
+    V_CG_P = 11,      // p = z + beta*p                          (z: v)
+    V_COPY_DOT = 12,  // p = z ; partial (r, z)                  (z: v)
+    V_COPY = 13,      // p = v
+    // RBiCGStab steps 16, 23-25 (rbicgstab.inl:186-216) and merged_rbicgstab_group2
+    // (blas_merged_rbicgstab.inl:120-196): x:x, v_:p, z:s, r:r, t:t, t_:p_, s:v
+    V_RB_UPDATE = 14,
+    V_RB_UPDATE_MERGED = 15,
 };
 
 struct VecArgs {
@@ -185,7 +197,20 @@ enum FinOp : int {
     FIN_OMEGA = 2,  // phi, psi, theta ; omega ; res ; ac, oc ; convergence check
     FIN_BETA = 3,   // rho_new ; beta ; pcoef ; rho = rho_new ; max_iters check
     FIN_OMEGA_MERGED = 4, // as FIN_OMEGA, but ac/oc recomputed with the updated iconv
+    FIN_CG_RHO = 5,   // rho = (r, z)                                        (PCG start)
+    FIN_CG_ALPHA = 6, // ++iters ; alpha = rho / (p, Ap) ; nalpha ; ac
+    FIN_CG_BETA = 7,  // beta = (r,z)/rho ; rho = (r,z) ; res = (r,r) ; check ; max_iters
+    FIN_RB_OMEGA = 8, // theta, phi, psi, eta ; omega ; res ; ac, oc ; check ; rho ; beta
+    FIN_RB_OMEGA_MERGED = 9, // as FIN_RB_OMEGA, ac/oc recomputed with the updated iconv
+    FIN_END = 10,     // done when the last check converged every RHS or at max_iters
 };
+// dots reduced by each finalize op
+__host__ __device__ inline int fin_ndots(int op) {
+    return (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) ? 3
+           : (op == FIN_CG_BETA)                       ? 2
+           : (op == FIN_RB_OMEGA || op == FIN_RB_OMEGA_MERGED) ? 4
+                                                       : 1;
+}
 void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int nv,
               hipStream_t s);
 

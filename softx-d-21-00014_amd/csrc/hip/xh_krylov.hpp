@@ -5,7 +5,9 @@
 // References: base_solver::convergence_check / converged (src/solvers/solver.h:173-241),
 // xdivy_z / ax_y / axpby on [1][NV] vectors (src/blas/detail/blas_ext.inl:194-239,
 // src/blas/detail/blas.inl:31-151), the scalar steps of PBiCGStab / MergedPBiCGStab
-// (src/solvers/detail/pbicgstab.inl:141-230, merged/merged_pbicgstab.inl:129-221).
+// (src/solvers/detail/pbicgstab.inl:141-230, merged/merged_pbicgstab.inl:129-221), PCG
+// (src/solvers/detail/pcg.inl:129-160) and RBiCGStab / MergedRBiCGStab
+// (src/solvers/detail/rbicgstab.inl:133-216, merged/merged_rbicgstab.inl:133-216).
 #pragma once
 
 #include "xh_internal.hpp"
@@ -37,11 +39,11 @@ __device__ inline void finalize_body(int op, SolverState *__restrict__ st,
                                      int stride) {
     if (st->done)
         return;
-    const int nd = (op == FIN_OMEGA || op == FIN_OMEGA_MERGED) ? 3 : 1;
+    const int nd = fin_ndots(op);
     const int C = nd * nv;
-    const int S = 256 / C; // >= 1 since nd*nv <= 3*64 < 256
+    const int S = 256 / C; // >= 1 since nd*nv <= 4*64 = 256
     __shared__ double red[256];
-    __shared__ double tot[3 * kMaxNV];
+    __shared__ double tot[4 * kMaxNV];
     __shared__ int s_notconv;
     const int tid = threadIdx.x;
     if (tid == 0)
@@ -136,6 +138,103 @@ __device__ inline void finalize_body(int op, SolverState *__restrict__ st,
                 st->ac[k] = st->iconv[k] * st->alpha[k];
                 st->oc[k] = st->iconv[k] * st->omega[k];
             }
+        }
+        return;
+    }
+    if (op == FIN_CG_RHO) {
+        if (act)
+            st->rho[k] = tot[k]; // dot(r, z, scal1) of the first iteration
+        return;
+    }
+    if (op == FIN_CG_ALPHA) {
+        if (tid == 0)
+            st->iters += 1;
+        if (act) {
+            const double al = st->rho[k] / tot[k]; // xdivy_z(scal1, scal2, alpha)
+            st->alpha[k] = al;
+            st->ac[k] = st->iconv[k] * al;         // ax_y(iconv, alpha, alpha_conv)
+            st->nalpha[k] = al * -1.0;             // scal(a_1, alpha)
+        }
+        return;
+    }
+    if (op == FIN_CG_BETA) {
+        if (act) {
+            const double s3 = tot[k];
+            st->beta[k] = s3 / st->rho[k]; // xdivy_z(scal3, scal1, beta)
+            st->rho[k] = s3;               // next iteration's dot(r, z, scal1)
+            st->res[k] = tot[nv + k];      // dot(r, r, res)
+            const int it = st->iters;
+            if (st->hist && it < st->hist_len)
+                st->hist[(size_t)it * nv + k] = sqrt(st->res[k]);
+        }
+        __syncthreads();
+        if (st->conv_check) {
+            conv_check(st, st->res, st->rho0, k, nv, &s_notconv);
+            __syncthreads();
+            if (!s_notconv) {
+                if (tid == 0) {
+                    st->conv_all = 1;
+                    st->done = 1;
+                    st->t_done = wall_clock64();
+                }
+                return;
+            }
+            if (act)
+                st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
+        }
+        if (tid == 0 && st->iters >= st->max_iters) {
+            st->done = 1;
+            st->t_done = wall_clock64();
+        }
+        return;
+    }
+    if (op == FIN_RB_OMEGA || op == FIN_RB_OMEGA_MERGED) {
+        if (act) {
+            const double th = tot[k], ph = tot[nv + k], ps = tot[2 * nv + k],
+                         et = tot[3 * nv + k];
+            st->theta[k] = th;
+            st->phi[k] = ph;
+            st->psi[k] = ps;
+            const double om = th / ph;   // xdivy_z(theta, phi, omega)
+            st->omega[k] = om;
+            st->nomega[k] = om * -1.0;
+            const double wr = om * th;   // ax_y(omega, theta, res)
+            st->res[k] = et + -1.0 * wr; // axpby(a1, eta, a_1, res)
+            st->ac[k] = st->iconv[k] * st->alpha[k];
+            st->oc[k] = st->iconv[k] * om;
+            const int it = st->iters;
+            if (st->hist && it < st->hist_len)
+                st->hist[(size_t)it * nv + k] = sqrt(st->res[k]);
+            // steps 21-22, used only when the solve goes on
+            double rh = om * ps;         // ax_y(omega, psi, rho)
+            rh = rh * -1.0;              // scal(a_1, rho)
+            const double tp = st->scal1[k] * om; // ax_y(delta, omega, temp)
+            st->beta[k] = rh / tp;       // xdivy_z(rho, temp, beta)
+            st->rho[k] = rh;
+        }
+        __syncthreads();
+        if (!st->conv_check)
+            return;
+        conv_check(st, st->res, st->rho0, k, nv, &s_notconv);
+        __syncthreads();
+        if (!s_notconv) {
+            if (tid == 0)
+                st->conv_all = 1;
+            return;
+        }
+        if (act) {
+            st->iconv[k] = st->iconv[k] * (double)(!st->conv[k]);
+            if (op == FIN_RB_OMEGA_MERGED) {
+                st->ac[k] = st->iconv[k] * st->alpha[k];
+                st->oc[k] = st->iconv[k] * st->omega[k];
+            }
+        }
+        return;
+    }
+    if (op == FIN_END) {
+        if (tid == 0 && (st->conv_all || st->iters >= st->max_iters)) {
+            st->done = 1;
+            st->t_done = wall_clock64();
         }
         return;
     }

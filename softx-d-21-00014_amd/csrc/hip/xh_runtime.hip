@@ -569,9 +569,59 @@ void precond(xh_solver_s *S, const double *b, double *x) {
     smooth(S, pr, L0, L0.cheb_pre, x, b, true);
 }
 
-bool is_preconditioned(int m) { return m == XH_PBICGSTAB || m == XH_MERGED_PBICGSTAB; }
+bool is_preconditioned(int m) {
+    return m == XH_PBICGSTAB || m == XH_MERGED_PBICGSTAB || m == XH_PCG || m == XH_RBICGSTAB ||
+           m == XH_MERGED_RBICGSTAB;
+}
+
+bool is_rbicgstab(int m) { return m == XH_RBICGSTAB || m == XH_MERGED_RBICGSTAB; }
+
+// Vector roles of PCG (pcg.inl:77-80): r, z -> p_, p, Ap -> v.
+// RBiCGStab (rbicgstab.inl:85-92): r0, r, z -> p_, v_ -> p, v, s, t_ -> s_, t.
+
+// get_residual + z = M^{-1} r + p = z + rho = (r, z)   (pcg.inl:84-96, 114-115)
+void enqueue_init_pcg(xh_solver_s *S, const double *b, double *x) {
+    SpmvArgs a = args(S);
+    a.b = b;
+    a.out2 = S->t; // scratch copies, overwritten before use
+    a.out3 = S->s;
+    a.partials = S->partials;
+    int np = spmv(S->A0, S->nv, x, S->r, OP_RESID0, a, S->stream);
+    finalize(FIN_INIT, S->st, S->partials, np, S->nv, S->stream);
+    precond(S, S->r, S->p_);
+    VecArgs vc;
+    vc.st = S->st;
+    vc.p = S->p;
+    vc.v = S->p_;
+    vc.r = S->r;
+    vc.partials = S->partials;
+    np = vec_op(V_COPY_DOT, S->n, S->nv, vc, S->stream);
+    finalize(FIN_CG_RHO, S->st, S->partials, np, S->nv, S->stream);
+}
+
+// r0 = b - A x ; r = r0 ; rho = rho0 ; z = M^{-1} r0 ; v_ = z   (rbicgstab.inl:96-113)
+void enqueue_init_rbicgstab(xh_solver_s *S, const double *b, double *x) {
+    SpmvArgs a = args(S);
+    a.b = b;
+    a.out2 = S->r;
+    a.out3 = S->t; // scratch copy, overwritten before use
+    a.partials = S->partials;
+    int np = spmv(S->A0, S->nv, x, S->r0, OP_RESID0, a, S->stream);
+    finalize(FIN_INIT, S->st, S->partials, np, S->nv, S->stream);
+    precond(S, S->r0, S->p_);
+    VecArgs vc;
+    vc.st = S->st;
+    vc.p = S->p;
+    vc.v = S->p_;
+    vec_op(V_COPY, S->n, S->nv, vc, S->stream);
+}
 
 void enqueue_init(xh_solver_s *S, const double *b, double *x) {
+    const int m = S->prm.solver.method;
+    if (m == XH_PCG)
+        return enqueue_init_pcg(S, b, x);
+    if (is_rbicgstab(m))
+        return enqueue_init_rbicgstab(S, b, x);
     SpmvArgs a = args(S);
     a.b = b;
     a.out2 = S->p;
@@ -595,8 +645,82 @@ int enqueue_replacement(xh_solver_s *S, double *x) {
     return vec_op(V_DOT1, S->n, S->nv, vd, S->stream);
 }
 
+// one PCG iteration (pcg.inl:101-160)
+void enqueue_iteration_pcg(xh_solver_s *S, double *x) {
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    SpmvArgs a = args(S);
+    a.o1 = S->p;
+    a.partials = S->partials;
+    int np = spmv(S->A0, nv, S->p, S->v, OP_DOT1, a, st); // Ap ; (p, Ap)
+    finalize(FIN_CG_ALPHA, S->st, S->partials, np, nv, st);
+    VecArgs vx;
+    vx.st = S->st;
+    vx.x = x;
+    vx.p_ = S->p;
+    vx.t = S->v;
+    vx.r = S->r;
+    vec_op(V_CG_XR, S->n, nv, vx, st);
+    precond(S, S->r, S->p_);
+    VecArgs vd;
+    vd.st = S->st;
+    vd.r = S->r;
+    vd.r0 = S->p_;
+    vd.partials = S->partials;
+    np = vec_op(V_DOT2, S->n, nv, vd, st); // (r, z), (r, r)
+    finalize(FIN_CG_BETA, S->st, S->partials, np, nv, st);
+    VecArgs vp;
+    vp.st = S->st;
+    vp.p = S->p;
+    vp.v = S->p_;
+    vec_op(V_CG_P, S->n, nv, vp, st);
+}
+
+// one RBiCGStab / MergedRBiCGStab iteration (rbicgstab.inl:124-216,
+// merged_rbicgstab.inl:124-216)
+void enqueue_iteration_rbicgstab(xh_solver_s *S, double *x, bool merged) {
+    const int nv = S->nv;
+    hipStream_t st = S->stream;
+    SpmvArgs a = args(S);
+    a.o1 = S->r0;
+    a.partials = S->partials;
+    int np = spmv(S->A0, nv, S->p, S->v, OP_DOT1, a, st); // 6-7: v = A v_ ; delta
+    finalize(FIN_ALPHA, S->st, S->partials, np, nv, st);  // 9: alpha = rho / delta
+    precond(S, S->v, S->s);                               // 8: s = M^{-1} v
+    VecArgs vt;
+    vt.st = S->st;
+    vt.s = S->s_;
+    vt.r = S->p_;
+    vt.v = S->s;
+    vec_op(V_S_UPDATE, S->n, nv, vt, st);                 // 10: t_ = z - alpha s
+    SpmvArgs a4 = args(S);
+    a4.b = S->v;
+    a4.o1 = S->r0;
+    a4.r = S->st->alpha;
+    a4.out2 = S->r;
+    a4.partials = S->partials;
+    np = spmv(S->A0, nv, S->s_, S->t, OP_RDOT4, a4, st);  // 11-13
+    finalize(merged ? FIN_RB_OMEGA_MERGED : FIN_RB_OMEGA, S->st, S->partials, np, nv, st);
+    precond(S, S->t, S->p_);                              // 14: z = M^{-1} t
+    VecArgs vu;
+    vu.st = S->st;
+    vu.x = x;
+    vu.p = S->p;
+    vu.s = S->p_;
+    vu.r = S->r;
+    vu.t = S->t;
+    vu.p_ = S->s_;
+    vu.v = S->s;
+    vec_op(merged ? V_RB_UPDATE_MERGED : V_RB_UPDATE, S->n, nv, vu, st); // 16, 23-25
+    finalize(FIN_END, S->st, S->partials, 0, nv, st);
+}
+
 void enqueue_iteration(xh_solver_s *S, double *x, bool replace) {
     const int m = S->prm.solver.method;
+    if (m == XH_PCG)
+        return enqueue_iteration_pcg(S, x);
+    if (is_rbicgstab(m))
+        return enqueue_iteration_rbicgstab(S, x, m == XH_MERGED_RBICGSTAB);
     const int nv = S->nv;
     hipStream_t st = S->stream;
     const bool pre = is_preconditioned(m);
@@ -1019,15 +1143,17 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
         return fail(XH_EINVAL, "xh_solver_create: bad arguments (nv must be 1,2,4,...,64)");
     const int m = prm->solver.method;
     if (m != XH_BICGSTAB && m != XH_PBICGSTAB && m != XH_MERGED_BICGSTAB &&
-        m != XH_MERGED_PBICGSTAB)
-        return fail(XH_ENOTSUP, "xh_solver_create: solver method must be a BiCGStab variant");
+        m != XH_MERGED_PBICGSTAB && m != XH_PCG && m != XH_RBICGSTAB && m != XH_MERGED_RBICGSTAB)
+        return fail(XH_ENOTSUP, "xh_solver_create: solver method must be a Krylov method "
+                                "(PCG or a BiCGStab variant)");
     if (is_preconditioned(m) && prm->precond.method == XH_NONE)
         return fail(XH_EINVAL, "xh_solver_create: preconditioned method without preconditioner");
     if (prm->true_res_period < 0)
         return fail(XH_EINVAL, "xh_solver_create: true_res_period must be >= 0");
-    if (prm->true_res_period > 0 && m == XH_MERGED_BICGSTAB)
-        return fail(XH_ENOTSUP, "xh_solver_create: true_res_period is not available for "
-                                "MergedBiCGStab (x is updated after r)");
+    if (prm->true_res_period > 0 &&
+        !(m == XH_BICGSTAB || m == XH_PBICGSTAB || m == XH_MERGED_PBICGSTAB))
+        return fail(XH_ENOTSUP, "xh_solver_create: true_res_period is available for BiCGStab, "
+                                "PBiCGStab and MergedPBiCGStab only");
     const int nl = H->nlevels;
     for (int l = 0; l < nl; ++l)
         if (H->A[l].nrows == 0 || (l + 1 < nl && H->P[l].ncols != H->A[l + 1].nrows))
@@ -1159,7 +1285,7 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
             return bail(rc);
     }
     if ((rc = alloc_vec(S, &S->st, 1)) ||
-        (rc = alloc_vec(S, &S->partials, (size_t)kMaxPartials * 3 * kMaxNV)))
+        (rc = alloc_vec(S, &S->partials, (size_t)kMaxPartials * 4 * kMaxNV)))
         return bail(rc);
     S->hist_len = std::max(1, prm->solver.max_iters) + 2;
     if ((rc = alloc_vec(S, &S->hist, (size_t)S->hist_len * nv)))

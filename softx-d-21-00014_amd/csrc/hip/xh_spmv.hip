@@ -256,7 +256,10 @@ __device__ __forceinline__ void block_partials(double (&p)[ND][W], double *__res
 
 template <int OP>
 struct NDots {
-    static constexpr int value = (OP == OP_DOT1 || OP == OP_RESID0) ? 1 : (OP == OP_DOT3 ? 3 : 0);
+    static constexpr int value = (OP == OP_DOT1 || OP == OP_RESID0) ? 1
+                                 : OP == OP_DOT3                     ? 3
+                                 : OP == OP_RDOT4                    ? 4
+                                                                     : 0;
 };
 
 template <typename VT, int NV>
@@ -410,6 +413,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
                 part[2][w] = part[2][w] + (double)(sv.v[w] * sv.v[w]);
             }
             st<VT, W>((VT *)yv + o, yo);
+        } else if constexpr (OP == OP_RDOT4) {
+            // RBiCGStab 12-13 (merged_rbicgstab_group1): r^ = r - alpha v, then
+            // theta = (t, r^), phi = (t, t), psi = (t, r0), eta = (r^, r^)
+            const P vv = ld<VT, W>(b + o), r0v = ld<VT, W>(o1 + o), rv = ld<VT, W>(out2 + o);
+            P yo, rn;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                yo.v[w] = acc[w];
+                rn.v[w] = rv.v[w] - rr[k0 + w] * vv.v[w];
+                part[0][w] = part[0][w] + (double)(acc[w] * rn.v[w]);
+                part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
+                part[2][w] = part[2][w] + (double)(acc[w] * r0v.v[w]);
+                part[3][w] = part[3][w] + (double)(rn.v[w] * rn.v[w]);
+            }
+            st<VT, W>((VT *)yv + o, yo);
+            st<VT, W>(out2 + o, rn);
         } else if constexpr (OP == OP_CHEB0) {
             // r = b - A x ; r *= isd ; t1 = c*r ; then t1 *= isd (u) or x (+)= t1*isd
             const VT di = isd[i];
@@ -580,6 +599,8 @@ int by_op(const Mat &A, const void *x, void *y, SpmvOp op, const SpmvArgs &a, hi
             return launch_t<VT, IT, NV, OP_DOT3>(A, x, y, a, s);
         case OP_RESID0:
             return launch_t<VT, IT, NV, OP_RESID0>(A, x, y, a, s);
+        case OP_RDOT4:
+            return launch_t<VT, IT, NV, OP_RDOT4>(A, x, y, a, s);
         default:
             break;
         }

@@ -4,6 +4,7 @@
 //
 // References: blas::axpby_z / axpbypcz / xdivy_z (src/blas/detail/blas_ext.inl:31-239),
 // merged_bicgstab_group1/2/4/5 (src/blas/detail/specific/blas_merged_bicgstab.inl:33-257),
+// merged_rbicgstab_group2 (src/blas/detail/specific/blas_merged_rbicgstab.inl:120-196),
 // base_solver::convergence_check / converged (src/solvers/solver.h:173-241),
 // merged_jacobi (src/blas/detail/specific/blas_merged_jacobi.inl:33-90),
 // csr_SGS (src/blas2/detail/csr_gs.inl:33-134), dense_Ax_y
@@ -67,9 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double
         return;
     if (gate == GATE_CONV_ALL && !st->conv_all)
         return;
-    constexpr int ND = (OP == V_XR_UPDATE || OP == V_R_UPDATE || OP == V_DOT1) ? 1
-                       : (OP == V_DOT3)                                         ? 3
-                                                                                : 0;
+    constexpr int ND =
+        (OP == V_XR_UPDATE || OP == V_R_UPDATE || OP == V_DOT1 || OP == V_COPY_DOT) ? 1
+        : (OP == V_DOT2)                                                             ? 2
+        : (OP == V_DOT3)                                                             ? 3
+                                                                                     : 0;
     constexpr int RPB = kBlock / NV;
     const int64_t lb = logical_block();
     const int64_t start = lb * chunk;
@@ -98,7 +101,26 @@ __global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double
         c1 = st->ac[k];
         c2 = st->oc[k];
         c3 = st->beta[k] * st->omega[k];
+    } else if constexpr (OP == V_CG_XR) {
+        c1 = st->ac[k];
+        c2 = st->nalpha[k];
+    } else if constexpr (OP == V_CG_P) {
+        c1 = st->beta[k];
+    } else if constexpr (OP == V_RB_UPDATE) {
+        c1 = st->ac[k];
+        c2 = st->oc[k];
+        c3 = st->nomega[k];
+    } else if constexpr (OP == V_RB_UPDATE_MERGED) {
+        c1 = st->ac[k];
+        c2 = st->oc[k];
+        c3 = st->nomega[k];
     }
+    // RBiCGStab step 25 coefficients: (-omega)*beta (ax_y(omega, beta, delta) with omega
+    // negated) for the plain form, beta*omega for merged_rbicgstab_group2
+    const double be = (OP == V_RB_UPDATE || OP == V_RB_UPDATE_MERGED) ? st->beta[k] : 0.0;
+    const double d2 = (OP == V_RB_UPDATE) ? st->nomega[k] * be : 0.0;
+    const double om = (OP == V_RB_UPDATE_MERGED) ? st->omega[k] : 0.0;
+    const double bo = (OP == V_RB_UPDATE_MERGED) ? be * om : 0.0;
     for (int64_t i = start + tr; i < end; i += RPB) {
         const size_t o = (size_t)i * NV + k;
         if constexpr (OP == V_S_UPDATE) {
@@ -129,6 +151,34 @@ __global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double
             part[2] = part[2] + sv * sv;
         } else if constexpr (OP == V_DOT1) {
             part[0] = part[0] + r[o] * r0[o];
+        } else if constexpr (OP == V_CG_XR) {
+            // axpby(alpha_conv, p, a1, x) ; axpby(-alpha, Ap, a1, r)
+            x[o] = c1 * p_[o] + x[o];
+            r[o] = c2 * t[o] + r[o];
+        } else if constexpr (OP == V_DOT2) {
+            const double rv = r[o], zv = r0[o];
+            part[0] = part[0] + rv * zv;
+            part[1] = part[1] + rv * rv;
+        } else if constexpr (OP == V_CG_P) {
+            p[o] = v[o] + c1 * p[o]; // axpby(a1, z, beta, p)
+        } else if constexpr (OP == V_COPY_DOT) {
+            const double zv = v[o];
+            p[o] = zv;
+            part[0] = part[0] + r[o] * zv;
+        } else if constexpr (OP == V_COPY) {
+            p[o] = v[o];
+        } else if constexpr (OP == V_RB_UPDATE) {
+            r[o] = c3 * t[o] + r[o];                  // 16: axpby(-omega, t, a1, r)
+            x[o] = c1 * p[o] + c2 * p_[o] + x[o];     // 23: axpbypcz(ac, v_, oc, t_, a1, x)
+            const double zn = p_[o] + c3 * s[o];      // 24: axpby(a1, t_, -omega, z)
+            s[o] = zn;
+            p[o] = zn + d2 * v[o] + be * p[o];        // 25: axpbypcz(a1, z, delta, s, beta, v_)
+        } else if constexpr (OP == V_RB_UPDATE_MERGED) {
+            r[o] = c3 * t[o] + r[o];                  // 16
+            x[o] += c1 * p[o] + c2 * p_[o];           // group2: x += ac v_ + oc t_
+            const double zn = p_[o] - om * s[o];      //         z = t_ - omega z
+            s[o] = zn;
+            p[o] = zn + be * p[o] - bo * v[o];        //         v_ = z + beta v_ - beta omega s
         }
     }
     if constexpr (ND > 0)
@@ -138,7 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double
 template <int NV>
 int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
     const int rpb = kBlock / NV;
-    const bool dots = (op == V_XR_UPDATE || op == V_R_UPDATE || op == V_DOT1 || op == V_DOT3);
+    const bool dots = (op == V_XR_UPDATE || op == V_R_UPDATE || op == V_DOT1 || op == V_DOT3 ||
+                       op == V_DOT2 || op == V_COPY_DOT);
     int64_t need = (n + rpb - 1) / rpb;
     int64_t g = std::min<int64_t>(need, dots ? kMaxPartials : 8192);
     g = std::max<int64_t>(8, (g + 7) / 8 * 8);
@@ -160,6 +211,13 @@ int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
         XH_VEC_CASE(V_GROUP5)
         XH_VEC_CASE(V_DOT3)
         XH_VEC_CASE(V_DOT1)
+        XH_VEC_CASE(V_CG_XR)
+        XH_VEC_CASE(V_DOT2)
+        XH_VEC_CASE(V_CG_P)
+        XH_VEC_CASE(V_COPY_DOT)
+        XH_VEC_CASE(V_COPY)
+        XH_VEC_CASE(V_RB_UPDATE)
+        XH_VEC_CASE(V_RB_UPDATE_MERGED)
     default:
         return -1;
     }
@@ -172,7 +230,7 @@ int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
 __global__ __launch_bounds__(256) void k_finalize(int op, SolverState *__restrict__ st,
                                                   const double *__restrict__ partials,
                                                   int nparts, int nv) {
-    finalize_body(op, st, partials, nparts, nv, (op == FIN_OMEGA || op == FIN_OMEGA_MERGED ? 3 : 1) * nv);
+    finalize_body(op, st, partials, nparts, nv, fin_ndots(op) * nv);
 }
 
 // ---- smoother / coarse / conversion kernels ----

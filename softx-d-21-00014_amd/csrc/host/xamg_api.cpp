@@ -25,13 +25,19 @@ using xamg::amg::Hierarchy;
 using xamg::amg::SetupParams;
 
 namespace {
-
 thread_local std::string g_api_err;
+} // namespace
 
+namespace xamg {
 int api_fail(int code, const std::string &m) {
     g_api_err = m;
     return code;
 }
+} // namespace xamg
+
+namespace {
+
+using xamg::api_fail;
 
 uint64_t splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -344,7 +350,9 @@ int method_id(const std::string &m) {
         {"HSGS", XH_HSGS},                 {"Chebyshev", XH_CHEBYSHEV},
         {"Direct", XH_DIRECT},             {"MultiGrid", XH_MULTIGRID},
         {"BiCGStab", XH_BICGSTAB},         {"PBiCGStab", XH_PBICGSTAB},
-        {"MergedBiCGStab", XH_MERGED_BICGSTAB}, {"MergedPBiCGStab", XH_MERGED_PBICGSTAB}};
+        {"MergedBiCGStab", XH_MERGED_BICGSTAB}, {"MergedPBiCGStab", XH_MERGED_PBICGSTAB},
+        {"PCG", XH_PCG},                   {"RBiCGStab", XH_RBICGSTAB},
+        {"MergedRBiCGStab", XH_MERGED_RBICGSTAB}};
     auto it = ids.find(m);
     return it == ids.end() ? -100 : it->second;
 }

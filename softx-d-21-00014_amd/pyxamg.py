@@ -15,12 +15,14 @@ LIBDIR = os.path.join(HERE, "lib")
 # method ids, include/xamg_hip.h
 IDENTITY, JACOBI, HSGS, CHEBYSHEV, DIRECT, MULTIGRID = 0, 1, 2, 3, 4, 5
 BICGSTAB, PBICGSTAB, MERGED_BICGSTAB, MERGED_PBICGSTAB = 6, 7, 8, 9
+PCG, RBICGSTAB, MERGED_RBICGSTAB = 10, 11, 12
 NONE = -1
 
 METHODS = {
     "Identity": IDENTITY, "Jacobi": JACOBI, "HSGS": HSGS, "Chebyshev": CHEBYSHEV,
     "Direct": DIRECT, "MultiGrid": MULTIGRID, "BiCGStab": BICGSTAB, "PBiCGStab": PBICGSTAB,
     "MergedBiCGStab": MERGED_BICGSTAB, "MergedPBiCGStab": MERGED_PBICGSTAB,
+    "PCG": PCG, "RBiCGStab": RBICGSTAB, "MergedRBiCGStab": MERGED_RBICGSTAB,
 }
 
 
@@ -159,6 +161,19 @@ def _load():
     for f in ("xamg_cube_nnz", "xamg_varcoef7_nnz", "xamg_q1fem27_nnz"):
         getattr(_host, f).argtypes = [i64, i64, i64]
         getattr(_host, f).restype = i64
+    cp = C.c_char_p
+    for name, args in {
+        "xamg_io_read_header": [cp, vp, vp],
+        "xamg_io_sections": [cp, C.c_int, vp, vp],
+        "xamg_io_block_nnz": [cp, i64, i64, vp],
+        "xamg_io_read_system": [cp, C.c_int, i64, i64, vp, vp, vp, vp, vp, vp],
+        "xamg_io_write_system": [cp, i64, i64, vp, vp, vp, C.c_int, vp, vp],
+    }.items():
+        f = getattr(_host, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _host.xamg_io_block_range.argtypes = [i64, C.c_int, C.c_int, vp, vp]
+    _host.xamg_io_block_range.restype = None
     _host.xamg_amg_default_params.argtypes = [P(AmgParams)]
     _host.xamg_amg_default_params.restype = None
     _host.XAMG_get_last_error.restype = C.c_char_p
@@ -218,6 +233,48 @@ def varcoef7(nx, ny=None, nz=None, sigma=1.0, seed=0, kcell=None):
     check(host().xamg_varcoef7(nx, ny, nz, _ptr(k), _ptr(row), _ptr(col), _ptr(val)),
           "xamg_varcoef7")
     return row, col, val, k
+
+
+def io_block_range(nrows, nprocs, proc):
+    """the reference's per-rank row block (include/xamg_io.h): (row_begin, block_rows)"""
+    rb, nr = C.c_int64(), C.c_int64()
+    host().xamg_io_block_range(nrows, nprocs, proc, C.byref(rb), C.byref(nr))
+    return rb.value, nr.value
+
+
+def read_system(path, nv=1, nprocs=1, proc=0):
+    """one row block of a binary system file (include/xamg_io.h): dict(row, col, val, b, x,
+    nrows, found, row_begin); found bit 0: b from the file, bit 1: x from the file"""
+    bp = os.fsencode(path)
+    n, nnz = C.c_int64(), C.c_int64()
+    check(host().xamg_io_read_header(bp, C.byref(n), C.byref(nnz)), "xamg_io_read_header")
+    rb, nr = io_block_range(n.value, nprocs, proc)
+    bnnz = C.c_int64()
+    check(host().xamg_io_block_nnz(bp, rb, nr, C.byref(bnnz)), "xamg_io_block_nnz")
+    row = np.zeros(nr + 1, np.uint32)
+    col = np.zeros(max(bnnz.value, 1), np.uint32)
+    val = np.zeros(max(bnnz.value, 1))
+    b = np.zeros(max(nr * nv, 1))
+    x = np.zeros(max(nr * nv, 1))
+    found = C.c_int32()
+    check(host().xamg_io_read_system(bp, nv, rb, nr, _ptr(row), _ptr(col), _ptr(val), _ptr(b),
+                                     _ptr(x), C.byref(found)), "xamg_io_read_system")
+    k = bnnz.value
+    return dict(row=row, col=col[:k], val=val[:k], b=b[:nr * nv], x=x[:nr * nv],
+                nrows=n.value, found=found.value, row_begin=rb)
+
+
+def write_system(path, row, col, val, nv=1, b=None, x=None):
+    row = np.ascontiguousarray(row, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    val = np.ascontiguousarray(val, np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    xx = None if x is None else np.ascontiguousarray(x, np.float64)
+    check(host().xamg_io_write_system(os.fsencode(path), len(row) - 1, len(col), _ptr(row),
+                                      _ptr(col), _ptr(val), nv,
+                                      None if bb is None else _ptr(bb),
+                                      None if xx is None else _ptr(xx)),
+          "xamg_io_write_system")
 
 
 def q1fem27(nx, ny=None, nz=None, sigma=2.0, seed=0, kcell=None):

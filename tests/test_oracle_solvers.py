@@ -349,3 +349,118 @@ def test_true_residual_replacement_rejected_for_merged_bicgstab(O, X):
     p.true_res_period = 2
     with pytest.raises(RuntimeError):
         O.solve(O.Hier(levels), p, 1, X.rhs_uniform(0, 8 ** 3, 1))
+
+
+# ---- PCG, RBiCGStab, MergedRBiCGStab (pcg.inl, rbicgstab.inl, merged_rbicgstab.inl) ----
+
+def _krylov_basis(S, Minv, b, k):
+    """orthonormal basis of K_k(M^-1 S, M^-1 b)"""
+    v = Minv @ b
+    cols = []
+    for _ in range(k):
+        cols.append(v)
+        v = Minv @ (S @ v)
+    Q, _ = np.linalg.qr(np.array(cols).T)
+    return Q
+
+
+@pytest.mark.parametrize("precond", ["identity", "jacobi"])
+def test_pcg_iterate_is_the_krylov_energy_minimiser(O, precond):
+    # CG with an SPD preconditioner M: x_k minimises the S-norm error over
+    # x0 + K_k(M^-1 S, M^-1 r0); with x0 = 0 that is Q (Q^T S Q)^-1 Q^T b
+    n, nv, w = 30, 2, np.float32(0.7)
+    row, col, val, S = spd_csr(n, 11)
+    levels = [{"A": (n, n, row, col, val), "eig": (0.0, 1.0)}]
+    H = O.Hier(levels)
+    b = np.random.default_rng(12).standard_normal(n * nv)
+    B = b.reshape(-1, nv)
+    Minv = np.eye(n) if precond == "identity" else float(w) * np.diag(1 / np.diag(S))
+    for k in (1, 2, 4, 6):
+        p = krylov_params(O, O.PCG, O.IDENTITY if precond == "identity" else O.JACOBI,
+                          tol=0.0, iters=k)
+        p.precond.relax_factor = w
+        r = O.solve(H, p, nv, b)
+        assert r["iters"] == k
+        X = r["x"].reshape(-1, nv)
+        for c in range(nv):
+            Q = _krylov_basis(S, Minv, B[:, c], k)
+            xk = Q @ np.linalg.solve(Q.T @ S @ Q, Q.T @ B[:, c])
+            np.testing.assert_allclose(X[:, c], xk, rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("method", [10, 11, 12])
+def test_distinct_eigenvalues_bound_new_methods(O, method):
+    d = np.repeat([1.0, 2.0, 5.0, 11.0], 25)
+    H = O.Hier(diag_hier(d))
+    b = np.random.default_rng(4).standard_normal(100 * 3)
+    r = O.solve(H, krylov_params(O, method, O.IDENTITY, tol=1e-10), 3, b)
+    assert r["conv"].all() and r["iters"] <= 4
+    np.testing.assert_allclose(r["x"].reshape(-1, 3), b.reshape(-1, 3) / d[:, None], rtol=1e-8)
+
+
+def test_scaled_identity_new_methods(O):
+    # PCG on c I: alpha = 1/c, r = 0 after one step, x = b / c exactly
+    H = O.Hier(diag_hier(np.full(40, 2.0)))
+    b = np.random.default_rng(3).standard_normal(40 * 2)
+    r = O.solve(H, krylov_params(O, O.PCG, O.IDENTITY), 2, b)
+    assert r["iters"] == 1 and r["conv"].all()
+    np.testing.assert_array_equal(r["x"], b / 2.0)
+    # RBiCGStab: t_ = z - alpha s = 0, so omega = theta / phi = 0/0 as in BiCGStab
+    for m in (O.RBICGSTAB, O.MERGED_RBICGSTAB):
+        r = O.solve(H, krylov_params(O, m, O.IDENTITY), 2, b)
+        assert r["iters"] == 1 and r["conv"].all() and np.isnan(r["x"]).all()
+
+
+def test_rbicgstab_is_pbicgstab_reformulated(O, X):
+    # with a linear preconditioner RBiCGStab computes the iterates of right-preconditioned
+    # BiCGStab (v_ = M^-1 p, t_ = M^-1 s, r^ = s): equal residual histories up to rounding
+    row, col, val = O.cube(12)
+    levels = X.Amg(row, col, val).export()
+    H = O.Hier(levels)
+    b = X.rhs_uniform(0, 12 ** 3, 4)
+    out = {}
+    for m in (O.PBICGSTAB, O.RBICGSTAB, O.MERGED_RBICGSTAB):
+        p = mg_params(O, O.CHEBYSHEV, O.CHEBYSHEV, order=3)
+        p.solver = O.role(m, max_iters=50, rel_tol=1e-8, conv_check=1)
+        out[m] = O.solve(H, p, 4, b, hist_len=60)
+    ref = out[O.PBICGSTAB]
+    for m in (O.RBICGSTAB, O.MERGED_RBICGSTAB):
+        r = out[m]
+        assert r["conv"].all() and abs(r["iters"] - ref["iters"]) <= 1
+        k = min(5, len(r["hist"]), len(ref["hist"]))
+        np.testing.assert_allclose(r["hist"][:k], ref["hist"][:k], rtol=1e-8,
+                                   atol=1e-12 * ref["hist"][0].max())
+        # both stop at rel-res 1e-8, possibly one iteration apart
+        np.testing.assert_allclose(r["x"], ref["x"], rtol=0, atol=1e-4 * np.abs(ref["x"]).max())
+    # the merged and plain forms differ only in the association of the v_ update (and the
+    # x update of a RHS in the iteration it converges): not bit-identical, same path
+    a, c = out[O.RBICGSTAB], out[O.MERGED_RBICGSTAB]
+    np.testing.assert_allclose(a["hist"][:4], c["hist"][:4], rtol=1e-10,
+                               atol=1e-13 * a["hist"][0].max())
+    assert not np.array_equal(a["x"], c["x"])
+
+
+def test_pcg_solves_cube_with_amg(O, X):
+    N, nv = 12, 2
+    row, col, val = O.cube(N)
+    levels = X.Amg(row, col, val).export()
+    H = O.Hier(levels)
+    b = X.rhs_uniform(2, N ** 3, nv)
+    p = mg_params(O, O.JACOBI, O.JACOBI, relax=0.8, sweeps=2)
+    p.solver = O.role(O.PCG, max_iters=60, rel_tol=1e-10, conv_check=1)
+    r = O.solve(H, p, nv, b)
+    A = sp.csr_matrix((val, col, row), shape=(N ** 3, N ** 3))
+    xs = spla.spsolve(A.tocsc(), b.reshape(-1, nv))
+    assert r["conv"].all()
+    assert (np.abs(r["x"].reshape(-1, nv) - xs).max(0) / np.abs(xs).max(0)).max() < 1e-8
+
+
+def test_true_residual_rejected_for_new_methods(O, X):
+    row, col, val = O.cube(8)
+    levels = X.Amg(row, col, val).export()
+    for m in (O.PCG, O.RBICGSTAB, O.MERGED_RBICGSTAB):
+        p = mg_params(O, O.JACOBI, O.JACOBI)
+        p.solver = O.role(m, max_iters=50, rel_tol=1e-8, conv_check=1)
+        p.true_res_period = 5
+        with pytest.raises(RuntimeError):
+            O.solve(O.Hier(levels), p, 1, X.rhs_uniform(0, 8 ** 3, 1))
