@@ -9,7 +9,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "softx-d-21-00014_amd", "lib")
 HEADERS = {"xamg_hip.h": "libxamg_hip.so", "xamg_amg.h": "libxamg.so",
-           "xamg_c_api.h": "libxamg.so", "xamg_unet.h": "libxamg_hip.so"}
+           "xamg_c_api.h": "libxamg.so", "xamg_unet.h": "libxamg_hip.so",
+           "xamg_io.h": "libxamg.so"}
 
 
 def declared(header):
