@@ -107,6 +107,23 @@ int xh_spmv_timed(xh_matrix A, int nv, const void *dx, void *dy, int accumulate,
  * it is the largest one of at least a quarter of the rows and >= 256).
  * xh_matrix_fold: set >= 0 overrides it (0: plain row order), set < 0 only queries. */
 int xh_matrix_fold(xh_matrix A, int64_t set, int64_t *fold);
+
+/* Structured-grid view of a square matrix (set at upload when it applies): rows are the
+ * points of an nx x ny x nz grid in natural order (row = (z*ny + y)*nx + x) and every
+ * entry couples grid neighbours (|dx|, |dy|, |dz| <= 1; 7- and 27-point operators).  Such
+ * a matrix runs the SpMV as a sweep along x over tiles of grid lines with the x rows of
+ * the tile and its halo lines staged in LDS.  No reference counterpart (the reference's csr_Ax_y is row-parallel, src/blas2/
+ * detail/csr_matvec.inl:149-260); the arithmetic is that of the row-parallel kernel.
+ * xh_matrix_grid: dims = {nx, ny, nz}, zeros when the matrix has no grid view.
+ * xh_csr_grid: the same detection on a host CSR; code
+ * (nullable, nnz bytes) receives 9(dz+1) + 3(dy+1) + (dx+1) per entry or 255, coded the
+ * number of coded entries.  xh_grid_schedule: the work items of one launch, 4 values
+ * each (first line y, first line z, x begin, x end). */
+int xh_matrix_grid(xh_matrix A, int32_t *dims);
+int xh_csr_grid(int64_t nrows, const uint32_t *row, const uint32_t *col, int32_t *dims,
+                uint8_t *code, int64_t *coded);
+int xh_grid_schedule(int32_t nx, int32_t ny, int32_t nz, int prec, int nv, int dots,
+                     int64_t *grid, int64_t *nitems, int64_t *ipw, int32_t *items);
 /* the detection rule above on a host CSR matrix */
 int xh_csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_t *col,
                 int64_t *fold);

@@ -39,6 +39,11 @@ struct Mat {
     uint32_t *col_ind = nullptr; // ncols, or null
     int64_t col_offset = 0;
     int64_t y_rows = 0;          // rows of y (block rows before compression)
+    // structured-grid view (0: none): row i = (z*gy + y)*gx + c; gcode[j] names the
+    // neighbour (a, b, d) = (dz, dy, dx) of entry j as 9(a+1) + 3(b+1) + (d+1), or 255
+    // for a column that is not the grid neighbour row + a*gx*gy + b*gx + d
+    int32_t gx = 0, gy = 0, gz = 0;
+    uint8_t *gcode = nullptr;
     size_t bytes = 0;            // device bytes owned
 };
 
@@ -82,6 +87,38 @@ struct Launch {
     Sched sc;
 };
 
+// Work items of the structured-grid SpMV.  The gy x gz grid lines are cut into tiles of
+// ty x tz lines (ty*tz = rows per workgroup pass) and each line into nseg segments of
+// seglen rows; item k = (seg, zb, yb), yb fastest.  Workgroup w takes items
+// [w*ipw, (w+1)*ipw) and sweeps each along x.
+struct GridSched {
+    int32_t nx = 0, ny = 0, nz = 0, ty = 0, tz = 0, nyb = 0, nzb = 0, nseg = 1, seglen = 0;
+    int32_t nitems = 0, ipw = 1;
+};
+struct GridLaunch {
+    int grid = 0;
+    GridSched g;
+};
+// tile lines per workgroup pass of rpb rows: tz = 2^floor(log2(rpb)/2), ty = rpb / tz
+__host__ __device__ constexpr int grid_tz(int rpb) {
+    return rpb >= 256 ? 16 : rpb >= 64 ? 8 : rpb >= 16 ? 4 : rpb >= 4 ? 2 : 1;
+}
+GridLaunch grid_plan(int nx, int ny, int nz, int rpb, int cap);
+bool csr_grid(int64_t n, const uint32_t *row, const uint32_t *col, int32_t &nx, int32_t &ny,
+              int32_t &nz);
+int64_t grid_codes(int64_t n, const uint32_t *row, const uint32_t *col, int32_t nx, int32_t ny,
+                   int32_t nz, uint8_t *code);
+// item k -> tile (yb, zb) and x range [c0, c1)
+__host__ __device__ inline void grid_item(const GridSched &g, int k, int &yb, int &zb, int &c0,
+                                          int &c1) {
+    yb = k % g.nyb;
+    const int t = k / g.nyb;
+    zb = t % g.nzb;
+    const int seg = t / g.nzb;
+    c0 = seg * g.seglen;
+    c1 = c0 + g.seglen < g.nx ? c0 + g.seglen : g.nx;
+}
+
 // Tile schedule and grid of one SpMV launch over n rows with nnz nonzeros and fold F
 // (0: none): rpb rows per workgroup pass, at most cap workgroups, row_bytes bytes of x
 // per row.
@@ -99,6 +136,12 @@ struct Tuning {
     int tile_passes = 8;            // tile rows = tile_passes * rows per workgroup pass
     int fold_enable = 1;            // 0: always contiguous order
     int64_t fold_min_bytes = 2 << 20; // fold only when 2*fold*nv*sizeof(value) exceeds this
+    // 1: structured-grid matrices run the LDS-window line sweep (k_spmv_grid).  Off by
+    // default: at 128^3 it matches the row-parallel kernel at nv = 32 and is slower at
+    // other nv and in fp32
+    int grid_enable = 0;
+    int grid_items = 2048;          // target work items (tile x segment) per launch
+    int grid_min_seg = 32;          // shortest x segment of a work item
 };
 extern Tuning g_tune;
 

@@ -115,6 +115,78 @@ int64_t csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32
     return (bc * 4 >= nrows && best >= min_fold) ? best : 0;
 }
 
+// Grid view of a square matrix: rows = points of an nx x ny x nz grid in natural order.
+// nx and F = nx*ny are the most frequent pair of column offsets (sampled rows) with
+// F % nx == 0, nx >= 3, F / nx >= 3, n % F == 0 and each offset on at least a quarter
+// of the rows; both are the most frequent offsets of their neighbourhoods in a 7- or
+// 27-point operator.  False when no pair qualifies.
+bool csr_grid(int64_t n, const uint32_t *row, const uint32_t *col, int32_t &nx, int32_t &ny,
+              int32_t &nz) {
+    if (n < 27)
+        return false;
+    std::unordered_map<int64_t, int64_t> cnt;
+    const int64_t step = std::max<int64_t>(1, n / 65536);
+    int64_t sampled = 0;
+    for (int64_t i = 0; i < n; i += step, ++sampled)
+        for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
+            const int64_t d = (int64_t)col[j] - i;
+            if (d >= 3)
+                ++cnt[d];
+        }
+    std::vector<std::pair<int64_t, int64_t>> top(cnt.begin(), cnt.end());
+    std::sort(top.begin(), top.end(), [](const auto &p, const auto &q) {
+        return p.second != q.second ? p.second > q.second : p.first < q.first;
+    });
+    if (top.size() > 16)
+        top.resize(16);
+    int64_t bx = 0, bF = 0, bs = 0;
+    for (const auto &p : top)
+        for (const auto &q : top) {
+            const int64_t a = p.first, F = q.first;
+            // both offsets on at least a quarter of the rows, as in a grid operator
+            if (F <= a || F % a != 0 || F / a < 3 || n % F != 0 || F > INT32_MAX ||
+                4 * p.second < sampled || 4 * q.second < sampled)
+                continue;
+            const int64_t sc = p.second + q.second;
+            if (sc > bs) {
+                bs = sc;
+                bx = a;
+                bF = F;
+            }
+        }
+    if (bs == 0)
+        return false;
+    nx = (int32_t)bx;
+    ny = (int32_t)(bF / bx);
+    nz = (int32_t)(n / bF);
+    return true;
+}
+
+// gcode of every entry for the grid view (Mat::gcode); returns the coded count
+int64_t grid_codes(int64_t n, const uint32_t *row, const uint32_t *col, int32_t nx, int32_t ny,
+                   int32_t nz, uint8_t *code) {
+    const int64_t F = (int64_t)nx * ny;
+    int64_t coded = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c = i % nx, y = (i / nx) % ny, z = i / F;
+        for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
+            const int64_t dl = (int64_t)col[j] - i;
+            uint8_t cd = 255;
+            for (int a = -1; a <= 1 && cd == 255; ++a)
+                for (int b = -1; b <= 1 && cd == 255; ++b)
+                    for (int d = -1; d <= 1; ++d)
+                        if (dl == a * F + b * (int64_t)nx + d && c + d >= 0 && c + d < nx &&
+                            y + b >= 0 && y + b < ny && z + a >= 0 && z + a < nz) {
+                            cd = (uint8_t)(9 * (a + 1) + 3 * (b + 1) + (d + 1));
+                            break;
+                        }
+            code[j] = cd;
+            coded += cd != 255;
+        }
+    }
+    return coded;
+}
+
 namespace {
 
 template <typename T>
@@ -221,6 +293,20 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
     }
     if (h.nrows == h.ncols)
         M.fold = csr_fold(h.nrows, h.ncols, h.row.data(), h.col.data(), 256);
+    // structured-grid view when every entry is a grid neighbour
+    int32_t gx, gy, gz;
+    if (h.nrows == h.ncols && nnz > 0 && csr_grid(n, h.row.data(), h.col.data(), gx, gy, gz)) {
+        std::vector<uint8_t> code(nnz);
+        const int64_t coded = grid_codes(n, h.row.data(), h.col.data(), gx, gy, gz, code.data());
+        if (coded == nnz) {
+            if ((rc = dmalloc(&M.gcode, nnz, acc)))
+                return rc;
+            XH_HIP(hipMemcpy(M.gcode, code.data(), nnz, hipMemcpyHostToDevice));
+            M.gx = gx;
+            M.gy = gy;
+            M.gz = gz;
+        }
+    }
     M.y_rows = h.nrows;
     M.bytes = acc;
     return 0;
@@ -228,7 +314,7 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
 
 void free_mat(Mat &M) {
     for (void *p : {(void *)M.row, M.col, (void *)M.panel_base, M.val, M.inv_diag,
-                    M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind})
+                    M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind, (void *)M.gcode})
         if (p)
             (void)hipFree(p);
     M = Mat();
@@ -1445,6 +1531,12 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.fold_enable = (int)value;
     else if (k == "spmv_fold_min_bytes" && value >= 0)
         g_tune.fold_min_bytes = value;
+    else if (k == "spmv_grid_enable" && (value == 0 || value == 1))
+        g_tune.grid_enable = (int)value;
+    else if (k == "spmv_grid_items" && value >= 1 && value <= (1 << 24))
+        g_tune.grid_items = (int)value;
+    else if (k == "spmv_grid_min_seg" && value >= 1 && value <= (1 << 24))
+        g_tune.grid_min_seg = (int)value;
     else
         return fail(XH_EINVAL, "xh_set_option: unknown key or value out of range: " + k);
     return 0;
@@ -1477,6 +1569,61 @@ extern "C" int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int pr
     if (rows)
         for (int64_t k = 0; k < L.sc.ntiles; ++k)
             tile_rows(L.sc, k, rows[2 * k], rows[2 * k + 1]);
+    return 0;
+}
+
+extern "C" int xh_csr_grid(int64_t nrows, const uint32_t *row, const uint32_t *col,
+                           int32_t *dims, uint8_t *code, int64_t *coded) {
+    if (!row || !dims || (nrows > 0 && row[nrows] > 0 && !col))
+        return fail(XH_EINVAL, "xh_csr_grid: null pointer");
+    dims[0] = dims[1] = dims[2] = 0;
+    if (coded)
+        *coded = 0;
+    if (!csr_grid(nrows, row, col, dims[0], dims[1], dims[2]))
+        return 0;
+    std::vector<uint8_t> tmp;
+    if (!code) {
+        tmp.resize(row[nrows]);
+        code = tmp.data();
+    }
+    const int64_t c = grid_codes(nrows, row, col, dims[0], dims[1], dims[2], code);
+    if (coded)
+        *coded = c;
+    return 0;
+}
+
+extern "C" int xh_grid_schedule(int32_t nx, int32_t ny, int32_t nz, int prec, int nv, int dots,
+                                int64_t *grid, int64_t *nitems, int64_t *ipw, int32_t *items) {
+    if (nx < 1 || ny < 1 || nz < 1 || (prec != F64 && prec != F32) || !nv_ok(nv))
+        return fail(XH_EINVAL, "xh_grid_schedule: bad argument");
+    const int vb = prec == F64 ? 8 : 4;
+    const int w = std::min(16 / vb, nv);
+    const int rpb = kBlock / (nv / w);
+    const GridLaunch G = grid_plan(nx, ny, nz, rpb, dots ? kMaxPartials : 16384);
+    if (grid)
+        *grid = G.grid;
+    if (nitems)
+        *nitems = G.g.nitems;
+    if (ipw)
+        *ipw = G.g.ipw;
+    if (items)
+        for (int k = 0; k < G.g.nitems; ++k) {
+            int yb, zb, c0, c1;
+            grid_item(G.g, k, yb, zb, c0, c1);
+            items[4 * k] = yb * G.g.ty;
+            items[4 * k + 1] = zb * G.g.tz;
+            items[4 * k + 2] = c0;
+            items[4 * k + 3] = c1;
+        }
+    return 0;
+}
+
+extern "C" int xh_matrix_grid(xh_matrix A, int32_t *dims) {
+    if (!A || !dims)
+        return fail(XH_EINVAL, "xh_matrix_grid: null argument");
+    dims[0] = A->m.gx;
+    dims[1] = A->m.gy;
+    dims[2] = A->m.gz;
     return 0;
 }
 

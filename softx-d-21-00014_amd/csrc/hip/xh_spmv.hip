@@ -225,10 +225,10 @@ __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
 // Sum over the lanes that own the same RHS (lanes l and l + L, l + 2L, ... of a wave),
 // then over the 4 waves: one partial per (dot, RHS) per workgroup, fixed order.
 template <int NV, int W, int ND>
-__device__ __forceinline__ void block_partials(double (&p)[ND][W], double *__restrict__ out,
-                                               int64_t lb) {
+__device__ __forceinline__ void block_partials_in(double (&p)[ND][W], double *__restrict__ out,
+                                                  int64_t lb,
+                                                  double (&red)[kBlock / 64][ND][NV]) {
     constexpr int L = NV / W;
-    __shared__ double red[kBlock / 64][ND][NV];
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -254,6 +254,13 @@ __device__ __forceinline__ void block_partials(double (&p)[ND][W], double *__res
     }
 }
 
+template <int NV, int W, int ND>
+__device__ __forceinline__ void block_partials(double (&p)[ND][W], double *__restrict__ out,
+                                               int64_t lb) {
+    __shared__ double red[kBlock / 64][ND][NV];
+    block_partials_in<NV, W, ND>(p, out, lb, red);
+}
+
 template <int OP>
 struct NDots {
     static constexpr int value = (OP == OP_DOT1 || OP == OP_RESID0) ? 1
@@ -266,6 +273,181 @@ template <typename VT, int NV>
 struct Width {
     static constexpr int value = (16 / (int)sizeof(VT)) < NV ? (16 / (int)sizeof(VT)) : NV;
 };
+
+// Operands of the fused epilogue of one row, loaded by epi_load (right before the
+// epilogue in k_spmv, a few passes ahead in k_spmv_grid).
+template <typename VT, int W>
+struct EpiIn {
+    Pack<VT, W> a, b, c;
+    Pack<double, W> xd;
+    VT di;
+    VT al[W];
+};
+
+template <typename VT, int NV, int W, int OP>
+__device__ __forceinline__ void epi_load(int64_t i, size_t o, size_t oy, int k0,
+                                         EpiIn<VT, W> &in, const void *__restrict__ yv,
+                                         const VT *__restrict__ b, const VT *__restrict__ o1,
+                                         const VT *__restrict__ isd, const VT *__restrict__ rr,
+                                         const VT *__restrict__ out2,
+                                         const void *__restrict__ xupdv, int last, int x_zero) {
+    if constexpr (OP == OP_ADD) {
+        in.a = ld<VT, W>((const VT *)yv + oy);
+    } else if constexpr (OP == OP_RESID || OP == OP_OFFBLOCK || OP == OP_RESID0) {
+        in.a = ld<VT, W>(b + o);
+    } else if constexpr (OP == OP_DOT1 || OP == OP_DOT3) {
+        in.a = ld<VT, W>(o1 + o);
+    } else if constexpr (OP == OP_RDOT4) {
+        in.a = ld<VT, W>(b + o);
+        in.b = ld<VT, W>(o1 + o);
+        in.c = ld<VT, W>(out2 + o);
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            in.al[w] = rr[k0 + w];
+    } else if constexpr (OP == OP_CHEB0 || OP == OP_CHEB) {
+        in.di = isd[i];
+        if constexpr (OP == OP_CHEB0)
+            in.a = ld<VT, W>(b + o);
+        else
+            in.a = ld<VT, W>(rr + o);
+        if (last && !x_zero) {
+            in.b = ld<VT, W>((const VT *)xupdv + o);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                in.b.v[w] = VT(0);
+        }
+    } else if constexpr (OP == OP_PROLONG32) {
+        if (!x_zero)
+            in.xd = ld<double, W>((const double *)yv + o);
+    }
+}
+
+// The fused epilogue of one row: acc holds the row's W sums (y = A x, or y + A x for
+// OP_ADD); o is the row's offset in the nv-interleaved vectors, oy that of the output row;
+// `in` holds the operands epi_load read for the row.
+template <typename VT, int NV, int W, int OP, int NP>
+__device__ __forceinline__ void row_epilogue(
+    int64_t i, size_t o, size_t oy, int k0, const VT (&acc)[W], double (&part)[NP][W],
+    const EpiIn<VT, W> &in, void *__restrict__ yv, VT *__restrict__ out2,
+    VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last, int x_zero, int out_f32) {
+    using P = Pack<VT, W>;
+    if constexpr (OP == OP_SET || OP == OP_ADD) {
+        P yo;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            yo.v[w] = acc[w];
+        st<VT, W>((VT *)yv + oy, yo);
+    } else if constexpr (OP == OP_RESID || OP == OP_OFFBLOCK) {
+        const P &bv = in.a;
+        if (OP == OP_RESID && out_f32) {
+            Pack<float, W> t;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                t.v[w] = (float)(bv.v[w] - acc[w]);
+            st<float, W>((float *)yv + o, t);
+        } else {
+            P t;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                t.v[w] = bv.v[w] - acc[w];
+            st<VT, W>((VT *)yv + o, t);
+        }
+    } else if constexpr (OP == OP_RESID0) {
+        const P &bv = in.a;
+        P t;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            t.v[w] = bv.v[w] - acc[w];
+            part[0][w] = part[0][w] + (double)(t.v[w] * t.v[w]);
+        }
+        st<VT, W>((VT *)yv + o, t);
+        st<VT, W>(out2 + o, t);
+        st<VT, W>(out3 + o, t);
+    } else if constexpr (OP == OP_DOT1) {
+        const P &ov = in.a;
+        P yo;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            yo.v[w] = acc[w];
+            part[0][w] = part[0][w] + (double)(acc[w] * ov.v[w]);
+        }
+        st<VT, W>((VT *)yv + o, yo);
+    } else if constexpr (OP == OP_DOT3) {
+        const P &sv = in.a;
+        P yo;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            yo.v[w] = acc[w];
+            part[0][w] = part[0][w] + (double)(acc[w] * sv.v[w]);
+            part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
+            part[2][w] = part[2][w] + (double)(sv.v[w] * sv.v[w]);
+        }
+        st<VT, W>((VT *)yv + o, yo);
+    } else if constexpr (OP == OP_RDOT4) {
+        // RBiCGStab 12-13 (merged_rbicgstab_group1): r^ = r - alpha v, then
+        // theta = (t, r^), phi = (t, t), psi = (t, r0), eta = (r^, r^)
+        const P &vv = in.a, &r0v = in.b, &rv = in.c;
+        P yo, rn;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            yo.v[w] = acc[w];
+            rn.v[w] = rv.v[w] - in.al[w] * vv.v[w];
+            part[0][w] = part[0][w] + (double)(acc[w] * rn.v[w]);
+            part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
+            part[2][w] = part[2][w] + (double)(acc[w] * r0v.v[w]);
+            part[3][w] = part[3][w] + (double)(rn.v[w] * rn.v[w]);
+        }
+        st<VT, W>((VT *)yv + o, yo);
+        st<VT, W>(out2 + o, rn);
+    } else if constexpr (OP == OP_CHEB0) {
+        // r = b - A x ; r *= isd ; t1 = c*r ; then t1 *= isd (u) or x (+)= t1*isd
+        const VT di = in.di;
+        const P &bv = in.a;
+        P rv, t1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            rv.v[w] = bv.v[w] - acc[w];
+            rv.v[w] = rv.v[w] * di;
+            t1.v[w] = c * rv.v[w];
+            t1.v[w] = t1.v[w] * di;
+        }
+        // last: x (+)= t1 ; else r and u; one store through the selected pointer
+        if (last && !x_zero) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                t1.v[w] = t1.v[w] + in.b.v[w];
+        }
+        if (!last)
+            st<VT, W>(out2 + o, rv);
+        st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
+    } else if constexpr (OP == OP_CHEB) {
+        // t2 = (A u)*isd ; t1 = c*r + t2 ; then u' = t1*isd or x (+)= t1*isd
+        const VT di = in.di;
+        const P &r0 = in.a;
+        P t1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const VT t2 = acc[w] * di;
+            t1.v[w] = c * r0.v[w] + t2;
+            t1.v[w] = t1.v[w] * di;
+        }
+        if (last && !x_zero) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                t1.v[w] = t1.v[w] + in.b.v[w];
+        }
+        st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
+    } else if constexpr (OP == OP_PROLONG32) {
+        // VT = float: temp32 = P xH ; x64 = (double)temp32 + x64
+        double *xd = (double *)yv;
+        Pack<double, W> xo;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            xo.v[w] = x_zero ? (double)acc[w] : (double)acc[w] + in.xd.v[w];
+        st<double, W>(xd + o, xo);
+    }
+}
 
 template <typename VT, typename IT, int NV, int OP>
 __global__ __launch_bounds__(kBlock) void k_spmv(
@@ -361,133 +543,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         } else {
             row_acc<VT, IT, NV, W>(jb, je, ch, col, val, base, x, k0, acc);
         }
-        if constexpr (OP == OP_SET || OP == OP_ADD) {
-            P yo;
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                yo.v[w] = acc[w];
-            st<VT, W>((VT *)yv + oy, yo);
-        } else if constexpr (OP == OP_RESID || OP == OP_OFFBLOCK) {
-            const P bv = ld<VT, W>(b + o);
-            if (OP == OP_RESID && out_f32) {
-                Pack<float, W> t;
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    t.v[w] = (float)(bv.v[w] - acc[w]);
-                st<float, W>((float *)yv + o, t);
-            } else {
-                P t;
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    t.v[w] = bv.v[w] - acc[w];
-                st<VT, W>((VT *)yv + o, t);
-            }
-        } else if constexpr (OP == OP_RESID0) {
-            const P bv = ld<VT, W>(b + o);
-            P t;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                t.v[w] = bv.v[w] - acc[w];
-                part[0][w] = part[0][w] + (double)(t.v[w] * t.v[w]);
-            }
-            st<VT, W>((VT *)yv + o, t);
-            st<VT, W>(out2 + o, t);
-            st<VT, W>(out3 + o, t);
-        } else if constexpr (OP == OP_DOT1) {
-            const P ov = ld<VT, W>(o1 + o);
-            P yo;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                yo.v[w] = acc[w];
-                part[0][w] = part[0][w] + (double)(acc[w] * ov.v[w]);
-            }
-            st<VT, W>((VT *)yv + o, yo);
-        } else if constexpr (OP == OP_DOT3) {
-            const P sv = ld<VT, W>(o1 + o);
-            P yo;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                yo.v[w] = acc[w];
-                part[0][w] = part[0][w] + (double)(acc[w] * sv.v[w]);
-                part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
-                part[2][w] = part[2][w] + (double)(sv.v[w] * sv.v[w]);
-            }
-            st<VT, W>((VT *)yv + o, yo);
-        } else if constexpr (OP == OP_RDOT4) {
-            // RBiCGStab 12-13 (merged_rbicgstab_group1): r^ = r - alpha v, then
-            // theta = (t, r^), phi = (t, t), psi = (t, r0), eta = (r^, r^)
-            const P vv = ld<VT, W>(b + o), r0v = ld<VT, W>(o1 + o), rv = ld<VT, W>(out2 + o);
-            P yo, rn;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                yo.v[w] = acc[w];
-                rn.v[w] = rv.v[w] - rr[k0 + w] * vv.v[w];
-                part[0][w] = part[0][w] + (double)(acc[w] * rn.v[w]);
-                part[1][w] = part[1][w] + (double)(acc[w] * acc[w]);
-                part[2][w] = part[2][w] + (double)(acc[w] * r0v.v[w]);
-                part[3][w] = part[3][w] + (double)(rn.v[w] * rn.v[w]);
-            }
-            st<VT, W>((VT *)yv + o, yo);
-            st<VT, W>(out2 + o, rn);
-        } else if constexpr (OP == OP_CHEB0) {
-            // r = b - A x ; r *= isd ; t1 = c*r ; then t1 *= isd (u) or x (+)= t1*isd
-            const VT di = isd[i];
-            const P bv = ld<VT, W>(b + o);
-            P rv, t1;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                rv.v[w] = bv.v[w] - acc[w];
-                rv.v[w] = rv.v[w] * di;
-                t1.v[w] = c * rv.v[w];
-                t1.v[w] = t1.v[w] * di;
-            }
-            if (last) {
-                VT *xu = (VT *)xupdv;
-                if (!x_zero) {
-                    const P xo = ld<VT, W>(xu + o);
-#pragma unroll
-                    for (int w = 0; w < W; ++w)
-                        t1.v[w] = t1.v[w] + xo.v[w];
-                }
-                st<VT, W>(xu + o, t1);
-            } else {
-                st<VT, W>(out2 + o, rv);
-                st<VT, W>((VT *)yv + o, t1);
-            }
-        } else if constexpr (OP == OP_CHEB) {
-            // t2 = (A u)*isd ; t1 = c*r + t2 ; then u' = t1*isd or x (+)= t1*isd
-            const VT di = isd[i];
-            const P r0 = ld<VT, W>(rr + o);
-            P t1;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const VT t2 = acc[w] * di;
-                t1.v[w] = c * r0.v[w] + t2;
-                t1.v[w] = t1.v[w] * di;
-            }
-            if (last) {
-                VT *xu = (VT *)xupdv;
-                if (!x_zero) {
-                    const P xo = ld<VT, W>(xu + o);
-#pragma unroll
-                    for (int w = 0; w < W; ++w)
-                        t1.v[w] = t1.v[w] + xo.v[w];
-                }
-                st<VT, W>(xu + o, t1);
-            } else {
-                st<VT, W>((VT *)yv + o, t1);
-            }
-        } else if constexpr (OP == OP_PROLONG32) {
-            // VT = float: temp32 = P xH ; x64 = (double)temp32 + x64
-            double *xd = (double *)yv;
-            Pack<double, W> xo;
-            if (!x_zero)
-                xo = ld<double, W>(xd + o);
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                xo.v[w] = x_zero ? (double)acc[w] : (double)acc[w] + xo.v[w];
-            st<double, W>(xd + o, xo);
-        }
+        EpiIn<VT, W> in;
+        epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last, x_zero);
+        row_epilogue<VT, NV, W, OP>(i, o, oy, k0, acc, part, in, yv, out2, out3, xupdv, c, last,
+                                    x_zero, out_f32);
         jb = nb;
         je = ne;
         bs = nbs;
@@ -502,6 +561,273 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         block_partials<NV, W, ND>(part, partials, lb);
 }
 
+
+// ---- structured-grid SpMV ----
+// Rows that are the points of a gx x gy x gz grid (row = (z*gy + y)*gx + c) with columns
+// that are grid neighbours (|dz|, |dy|, |dx| <= 1: 7- and 27-point operators).  A
+// workgroup owns a tile of ty x tz grid lines (one row of each per pass, ty*tz = RPB) and
+// sweeps it along x.  LDS holds the x rows of the tile's lines plus one halo line on
+// each side, (ty+2) x (tz+2) lines, at four consecutive positions (a ring: c-1, c, c+1
+// read by pass c, c+2 written during it), so an x row is read from L2 about
+// (ty+2)(tz+2)/(ty*tz) times per launch instead of once per nonzero that names it.
+// The view applies when every entry is a grid neighbour (Mat::gcode < 27), so every x
+// read comes from the window.  The products and their order are those of k_spmv: the
+// results are identical.
+
+template <typename VT, int NV, int W>
+struct GChunk {
+    static constexpr int BW = 8;
+    static constexpr int L = NV / W;
+    static constexpr int LL = L < 16 ? L : 16;
+    // a 27-point row fits one chunk when the row has 2 or more lanes
+    static constexpr int E = L >= 2 ? (32 + LL - 1) / LL : BW;
+    static constexpr int CH = L >= 2 ? LL * E : BW;
+    uint32_t cd[E];
+    VT v[E];
+};
+
+template <typename VT, typename IT, int NV, int W>
+__device__ __forceinline__ void load_gchunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
+                                            const VT *__restrict__ val,
+                                            const uint8_t *__restrict__ gcode,
+                                            GChunk<VT, NV, W> &ch) {
+    // every lane loads, at an index clamped into the row (or to entry 0 for an empty
+    // range), so no load sits behind a branch; slots past the row end are never used
+    using C = GChunk<VT, NV, W>;
+    const int gl = C::L >= 2 ? (int)(threadIdx.x & 63) % C::L : 0;
+#pragma unroll
+    for (int q = 0; q < C::E; ++q) {
+        const uint32_t jj = j + (C::L >= 2 ? (uint32_t)(q * C::LL + gl) : (uint32_t)q);
+        const uint32_t jc = jj < e ? jj : (e > j ? e - 1 : 0u);
+        ch.cd[q] = (uint32_t)gcode[jc];
+        ch.v[q] = val[jc];
+    }
+}
+
+// an LDS (address space 3) object: accesses compile to ds_* instructions
+template <typename T>
+using lds_t = __attribute__((address_space(3))) T;
+
+template <typename VT, int W>
+__device__ __forceinline__ Pack<VT, W> lds_row(const lds_t<Pack<VT, W>> *src) {
+    Pack<VT, W> r;
+    if constexpr (W == 1) {
+        r.v[0] = *(const lds_t<VT> *)src;
+    } else {
+        typedef VT vec __attribute__((ext_vector_type(W)));
+        const vec t = *(const lds_t<vec> *)src;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            r.v[w] = t[w];
+    }
+    return r;
+}
+
+
+// acc[w] + sum over the entries [j, min(j + CH, e)) of `ch`, x from the LDS window.  The
+// window reads go in batches of 4, unconditionally (a slot past the row end holds the
+// row's last code, a valid window address), so a batch is in flight together; only the
+// accumulation of a slot past the row end is skipped.
+template <typename VT, typename IT, int NV, int W, int NLY, int NL, int L>
+__device__ __forceinline__ void consume_gchunk(uint32_t j, uint32_t e, const GChunk<VT, NV, W> &ch,
+                                               int pos, int lbase,
+                                               const lds_t<Pack<VT, W>> *win, int lane,
+                                               VT (&acc)[W]) {
+    using C = GChunk<VT, NV, W>;
+    constexpr int BW = 4; // 8 in flight costs occupancy and measured slower
+    auto addr = [&](uint32_t cd) __attribute__((always_inline)) {
+        const int a = (int)(cd / 9u) - 1, b = (int)((cd / 3u) % 3u) - 1, d = (int)(cd % 3u) - 1;
+        return win + ((((pos + d) & 3) * NL + lbase + a * NLY + b) * L + lane);
+    };
+    bool more = true; // uniform over the row's lanes
+    static_for<0, C::CH / BW>([&](auto bc) {
+        constexpr int b0 = decltype(bc)::value * BW;
+        more = more && (j + b0 < e);
+        if (!more)
+            return;
+        VT vq[BW];
+        Pack<VT, W> xv[BW];
+        static_for<0, BW>([&](auto qc) {
+            constexpr int t = b0 + decltype(qc)::value;
+            uint32_t cd;
+            if constexpr (C::L >= 2) {
+                cd = (uint32_t)bcast<C::L, t % C::LL>((int)ch.cd[t / C::LL]);
+                vq[t - b0] = bcast<C::L, t % C::LL>(ch.v[t / C::LL]);
+            } else {
+                cd = ch.cd[t];
+                vq[t - b0] = ch.v[t];
+            }
+            xv[t - b0] = lds_row<VT, W>(addr(cd));
+        });
+#pragma unroll
+        for (int q = 0; q < BW; ++q)
+            if (j + b0 + q < e)
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    acc[w] = acc[w] + xv[q].v[w] * vq[q];
+    });
+}
+
+template <typename VT, typename IT, int NV, int OP>
+__global__ __launch_bounds__(kBlock) void k_spmv_grid(
+    const uint32_t *__restrict__ row, const IT *__restrict__ col, const VT *__restrict__ val,
+    const uint8_t *__restrict__ gcode, const uint32_t *__restrict__ pbase, int shift,
+    const GridSched g, const VT *__restrict__ x, void *__restrict__ yv, const VT *__restrict__ b,
+    const VT *__restrict__ o1, const VT *__restrict__ isd, const VT *__restrict__ rr,
+    VT *__restrict__ out2, VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last,
+    int x_zero, int out_f32, double *__restrict__ partials, const int *__restrict__ done) {
+    constexpr int ND = NDots<OP>::value;
+    constexpr int W = Width<VT, NV>::value;
+    constexpr int L = NV / W;
+    constexpr int RPB = kBlock / L;
+    constexpr int TZ = grid_tz(RPB), TY = RPB / TZ;
+    constexpr int NLY = TY + 2, NL = NLY * (TZ + 2);
+    constexpr int UNITS = NL * L; // 16-byte pieces of one window position
+    constexpr int R = (UNITS + kBlock - 1) / kBlock;
+    using P = Pack<VT, W>;
+    using C = GChunk<VT, NV, W>;
+    // the window doubles as the buffer of the final dot reduction
+    union Shared {
+        P win[4][NL][L];
+        double red[kBlock / 64][ND > 0 ? ND : 1][NV];
+    };
+    __shared__ Shared sh;
+    const lds_t<P> *wr = (const lds_t<P> *)&sh.win[0][0][0];
+    if (done != nullptr && *done)
+        return;
+    const int64_t lb = logical_block();
+    const int grp = threadIdx.x / L, ty = grp % TY, tz = grp / TY;
+    const int lane = threadIdx.x % L, k0 = lane * W;
+    const int lbase = (tz + 1) * NLY + (ty + 1);
+    double part[ND > 0 ? ND : 1][W];
+#pragma unroll
+    for (int d = 0; d < (ND > 0 ? ND : 1); ++d)
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            part[d][w] = 0.0;
+    // One pass (one row of each tile line) uses what was loaded three passes earlier: the
+    // x rows of window position c + 2, the chunk of row c and the epilogue operands of
+    // row c, held in three register sets that rotate with the position.  A set also
+    // carries the row pointers of its next row (c + 3), loaded one round ahead of that
+    // row's chunk.
+    struct Set {
+        P s[R];
+        C ch;
+        EpiIn<VT, W> in;
+        uint32_t jb, je, nb, ne;
+    };
+    for (int it = 0; it < g.ipw; ++it) {
+        const int k = (int)lb * g.ipw + it;
+        if (k >= g.nitems)
+            break; // uniform over the workgroup
+        int yb, zb, c0, c1;
+        grid_item(g, k, yb, zb, c0, c1);
+        const int wy = yb * TY - 1, wz = zb * TZ - 1; // window origin (halo line)
+        const int yg = yb * TY + ty, zg = zb * TZ + tz;
+        const bool valid = yg < g.ny && zg < g.nz;
+        const int64_t rb = valid ? ((int64_t)zg * g.ny + yg) * g.nx : 0;
+        // position p of every window line: piece u = threadIdx.x + q*kBlock is lane u % L
+        // of line u / L; lines and positions outside the grid are never read
+        // every piece is loaded (at a row clamped into the grid, so no load sits behind a
+        // branch); pieces of lines or positions outside the grid are never read
+        auto stage_load = [&](int p, P (&sr)[R]) __attribute__((always_inline)) {
+            const int pc = p < 0 ? 0 : (p >= g.nx ? g.nx - 1 : p);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int u = (int)threadIdx.x + q * kBlock;
+                const int ln = u < UNITS ? u / L : NL - 1, lp = u % L;
+                int yy = wy + ln % NLY, zz = wz + ln / NLY;
+                yy = yy < 0 ? 0 : (yy >= g.ny ? g.ny - 1 : yy);
+                zz = zz < 0 ? 0 : (zz >= g.nz ? g.nz - 1 : zz);
+                sr[q] = ld<VT, W>(x + ((size_t)((int64_t)zz * g.ny + yy) * g.nx + pc) * NV +
+                                  lp * W);
+            }
+        };
+        auto stage_store = [&](int p, const P (&sr)[R]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int u = (int)threadIdx.x + q * kBlock;
+                if (u < UNITS)
+                    sh.win[p & 3][u / L][u % L] = sr[q];
+            }
+        };
+        // loads for row cc (chunk, epilogue operands), window position cc + 2 and the row
+        // pointers of row cc + 3; S.nb / S.ne hold the row pointers of row cc
+        // every load is issued (indices clamped into the line), so none sits behind a
+        // branch and the compiler keeps the three sets in flight
+        auto issue = [&](int cc, Set &S) __attribute__((always_inline)) {
+            const bool rv = valid && cc < c1;
+            S.jb = rv ? S.nb : 0u;
+            S.je = rv ? S.ne : 0u;
+            load_gchunk<VT, IT, NV, W>(S.jb, S.je, col, val, gcode, S.ch);
+            S.nb = row[rb + (cc + 3 < g.nx ? cc + 3 : g.nx)];
+            S.ne = row[rb + (cc + 4 < g.nx ? cc + 4 : g.nx)];
+            const int64_t i = rb + (cc < g.nx ? cc : g.nx - 1);
+            const size_t o = (size_t)i * NV + k0;
+            epi_load<VT, NV, W, OP>(i, o, o, k0, S.in, yv, b, o1, isd, rr, out2, xupdv, last,
+                                    x_zero);
+            stage_load(cc + 2, S.s);
+        };
+        // pass cc: rows of position cc (threads of rows outside the grid have an empty row
+        // and skip the epilogue), then the set is refilled for pass cc + 3
+        auto pass = [&](int cc, Set &S) __attribute__((always_inline)) {
+            const int64_t i = rb + cc;
+            const size_t o = (size_t)i * NV + k0;
+            const uint32_t jb = S.jb, je = S.je;
+            VT acc[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                acc[w] = OP == OP_ADD ? S.in.a.v[w] : VT(0);
+            consume_gchunk<VT, IT, NV, W, NLY, NL, L>(jb, je, S.ch, cc, lbase, wr, lane, acc);
+            if (je - jb > (uint32_t)C::CH)
+                for (uint32_t j = jb + C::CH; j < je; j += C::CH) {
+                    C ech;
+                    load_gchunk<VT, IT, NV, W>(j, je, col, val, gcode, ech);
+                    consume_gchunk<VT, IT, NV, W, NLY, NL, L>(j, je, ech, cc, lbase, wr, lane,
+                                                               acc);
+                }
+            if (valid)
+                row_epilogue<VT, NV, W, OP>(i, o, o, k0, acc, part, S.in, yv, out2, out3, xupdv,
+                                            c, last, x_zero, out_f32);
+            if (cc + 2 <= c1)
+                stage_store(cc + 2, S.s);
+            issue(cc + 3, S);
+            __syncthreads();
+        };
+        {
+            P t[R];
+            stage_load(c0 - 1, t);
+            stage_store(c0 - 1, t);
+            stage_load(c0, t);
+            stage_store(c0, t);
+            stage_load(c0 + 1, t);
+            stage_store(c0 + 1, t);
+        }
+        __syncthreads();
+        Set S0, S1, S2;
+        S0.nb = row[rb + c0];
+        S0.ne = row[rb + (c0 + 1 < g.nx ? c0 + 1 : g.nx)];
+        S1.nb = S0.ne;
+        S1.ne = row[rb + (c0 + 2 < g.nx ? c0 + 2 : g.nx)];
+        S2.nb = S1.ne;
+        S2.ne = row[rb + (c0 + 3 < g.nx ? c0 + 3 : g.nx)];
+        issue(c0, S0);
+        issue(c0 + 1, S1);
+        issue(c0 + 2, S2);
+        int cb = c0;
+        for (; cb + 3 <= c1; cb += 3) {
+            pass(cb, S0);
+            pass(cb + 1, S1);
+            pass(cb + 2, S2);
+        }
+        if (cb < c1)
+            pass(cb, S0);
+        if (cb + 1 < c1)
+            pass(cb + 1, S1);
+    }
+    if constexpr (ND > 0)
+        block_partials_in<NV, W, ND>(part, partials, lb, sh.red);
+}
 } // namespace
 
 // Tile schedule and grid of one launch: `rpb` rows per workgroup pass, at most `cap`
@@ -545,6 +871,28 @@ Launch spmv_plan(int64_t n, int64_t nnz, int64_t F, int rpb, int cap, int64_t ro
     return L;
 }
 
+
+GridLaunch grid_plan(int nx, int ny, int nz, int rpb, int cap) {
+    GridLaunch G;
+    GridSched &g = G.g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.tz = grid_tz(rpb);
+    g.ty = rpb / g.tz;
+    g.nyb = (ny + g.ty - 1) / g.ty;
+    g.nzb = (nz + g.tz - 1) / g.tz;
+    const int64_t tiles = (int64_t)g.nyb * g.nzb;
+    int64_t nseg = std::max<int64_t>(1, (g_tune.grid_items + tiles - 1) / tiles);
+    nseg = std::min<int64_t>(nseg, std::max(1, nx / std::max(1, g_tune.grid_min_seg)));
+    g.seglen = (int)((nx + nseg - 1) / nseg);
+    g.nseg = (nx + g.seglen - 1) / g.seglen;
+    g.nitems = (int)(tiles * g.nseg);
+    g.ipw = (g.nitems + cap - 1) / cap;
+    const int64_t grid = (g.nitems + g.ipw - 1) / g.ipw;
+    G.grid = (int)std::max<int64_t>(8, (grid + 7) / 8 * 8);
+    return G;
+}
 namespace {
 
 template <typename VT, typename IT, int NV, int OP>
@@ -553,6 +901,21 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
+    // The Chebyshev steps stay on the row-parallel kernel: their grid instantiation (x
+    // update through a runtime-selected pointer) does not compile with this toolchain.
+    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
+                  OP != OP_CHEB0 && OP != OP_CHEB) {
+        if (A.gcode != nullptr && g_tune.grid_enable && A.row_ind == nullptr) {
+            const GridLaunch G = grid_plan(A.gx, A.gy, A.gz, RPB, cap);
+            hipLaunchKernelGGL((k_spmv_grid<VT, IT, NV, OP>), dim3(G.grid), dim3(kBlock), 0, s,
+                               A.row, (const IT *)A.col, (const VT *)A.val, A.gcode,
+                               A.panel_base, A.panel_shift, G.g, (const VT *)x, y,
+                               (const VT *)a.b, (const VT *)a.o1, (const VT *)a.isd,
+                               (const VT *)a.r, (VT *)a.out2, (VT *)a.out3, a.xupd, (VT)a.c,
+                               a.last, a.x_zero, a.out_f32, a.partials, a.done);
+            return ND > 0 ? G.grid : 0;
+        }
+    }
     const Launch L = spmv_plan(A.nrows, A.nnz, A.fold, RPB, cap, (int64_t)NV * (int64_t)sizeof(VT));
     hipLaunchKernelGGL((k_spmv<VT, IT, NV, OP>), dim3(L.grid), dim3(kBlock), 0, s, A.row,
                        (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,

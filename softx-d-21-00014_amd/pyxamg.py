@@ -93,6 +93,10 @@ def _load():
         "xh_dense_matvec": [i64, C.c_int, C.c_int, vp, vp, vp],
         "xh_matrix_fold": [vp, i64, P(i64)],
         "xh_csr_fold": [i64, i64, vp, vp, P(i64)],
+        "xh_csr_grid": [i64, vp, vp, vp, vp, P(i64)],
+        "xh_grid_schedule": [C.c_int32, C.c_int32, C.c_int32, C.c_int, C.c_int, C.c_int,
+                             P(i64), P(i64), P(i64), vp],
+        "xh_matrix_grid": [vp, vp],
         "xh_spmv_schedule": [i64, i64, i64, C.c_int, C.c_int, C.c_int, P(i64), P(i64), P(i64),
                              vp],
         "xh_set_option": [C.c_char_p, i64],
@@ -331,6 +335,31 @@ def spmv_schedule(nrows, nnz, fold, nv, prec=0, dots=False):
     check(hip().xh_spmv_schedule(nrows, nnz, fold, prec, nv, int(dots), None, None, None,
                                  rows.ctypes.data), "xh_spmv_schedule")
     return g.value, tpw.value, rows
+
+
+def csr_grid(row, col):
+    """(dims (nx, ny, nz) or None, per-entry grid codes, coded count) of the structured-grid
+    view xh_matrix_create would attach (when coded >= 90 % of the entries)"""
+    n = len(row) - 1
+    row = np.ascontiguousarray(row, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    dims = np.zeros(3, np.int32)
+    code = np.full(max(len(col), 1), 255, np.uint8)
+    coded = C.c_int64(0)
+    check(hip().xh_csr_grid(n, _ptr(row), _ptr(col), _ptr(dims), _ptr(code),
+                            C.byref(coded)), "xh_csr_grid")
+    return (tuple(int(d) for d in dims) if dims[0] else None), code[:len(col)], coded.value
+
+
+def grid_schedule(nx, ny, nz, nv, prec=0, dots=False):
+    """(grid, items per workgroup, [nitems, 4] items: first line y, first line z, x0, x1)"""
+    g, ni, ipw = C.c_int64(), C.c_int64(), C.c_int64()
+    check(hip().xh_grid_schedule(nx, ny, nz, prec, nv, int(dots), C.byref(g), C.byref(ni),
+                                 C.byref(ipw), None), "xh_grid_schedule")
+    items = np.zeros((ni.value, 4), np.int32)
+    check(hip().xh_grid_schedule(nx, ny, nz, prec, nv, int(dots), None, None, None,
+                                 _ptr(items)), "xh_grid_schedule")
+    return g.value, ipw.value, items
 
 
 def set_option(key, value):
@@ -700,6 +729,12 @@ class Matrix:
         check(hip().xh_matrix_fold(self.m, -1 if set is None else int(set), C.byref(f)),
               "xh_matrix_fold")
         return f.value
+
+    def grid(self):
+        """(nx, ny, nz) of the structured-grid view, or None"""
+        d = np.zeros(3, np.int32)
+        check(hip().xh_matrix_grid(self.m, _ptr(d)), "xh_matrix_grid")
+        return tuple(int(v) for v in d) if d[0] else None
 
     def spmv_timed(self, nv, dx, dy, reps, accumulate=False):
         ms = C.c_double()

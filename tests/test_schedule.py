@@ -1,10 +1,12 @@
 """SpMV row schedule (host logic of libxamg_hip.so, no GPU): fold detection on
 structured and unstructured matrices, and the tile schedule covers every row exactly
 once with a grid the kernels accept (multiple of 8 workgroups, at most 1024 for
-launches that write dot partials)."""
+launches that write dot partials); the structured-grid view (dimensions and per-entry
+neighbour codes against a direct restatement) and its work items."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
+from conftest import random_csr
 
 
 def lap1(n):
@@ -105,3 +107,77 @@ def test_schedule_folds_when_band_exceeds_l2(X):
         X.set_option("spmv_fold_enable", 1)
     with pytest.raises(X.XamgError):
         X.set_option("no_such_knob", 1)
+
+
+# ---- structured-grid view (xh_csr_grid, xh_grid_schedule) ----
+
+def _numpy_codes(row, col, nx, ny, nz):
+    n = len(row) - 1
+    F = nx * ny
+    code = np.full(len(col), 255, np.uint8)
+    for i in range(n):
+        c, y, z = i % nx, (i // nx) % ny, i // F
+        for j in range(row[i], row[i + 1]):
+            dl = int(col[j]) - i
+            for a in (-1, 0, 1):
+                for b in (-1, 0, 1):
+                    for d in (-1, 0, 1):
+                        if dl == a * F + b * nx + d and 0 <= c + d < nx and 0 <= y + b < ny \
+                                and 0 <= z + a < nz:
+                            code[j] = 9 * (a + 1) + 3 * (b + 1) + (d + 1)
+    return code
+
+
+@pytest.mark.parametrize("gen,dims", [("cube", (10, 7, 5)), ("cube", (3, 3, 3)),
+                                      ("q1fem27", (6, 5, 4)), ("varcoef7", (9, 4, 6))])
+def test_grid_view_detection_and_codes(X, gen, dims):
+    if gen == "cube":
+        row, col, val = X.cube(*dims)
+    elif gen == "q1fem27":
+        row, col, val, _ = X.q1fem27(*dims)
+    else:
+        row, col, val, _ = X.varcoef7(*dims)
+    d, code, coded = X.csr_grid(row, col)
+    assert d == dims and coded == len(col)
+    np.testing.assert_array_equal(code, _numpy_codes(row, col, *dims))
+
+
+def test_grid_view_absent_for_unstructured_and_partial(X):
+    rng = np.random.default_rng(0)
+    row, col, val = random_csr(rng, 200, 200, 0.05)
+    assert X.csr_grid(row, col)[0] is None
+    # a grid operator with one off-grid coupling per row of the first plane: a view with
+    # coded < nnz, which the upload does not attach
+    row, col, val = X.cube(8, 6, 5)
+    n = len(row) - 1
+    r2, c2 = [0], []
+    for i in range(n):
+        cs = list(col[row[i]:row[i + 1]])
+        if i < 48:
+            cs.append((i + n // 2) % n)
+        c2 += cs
+        r2.append(len(c2))
+    d, code, coded = X.csr_grid(np.array(r2, np.uint32), np.array(c2, np.uint32))
+    assert d == (8, 6, 5) and coded == len(c2) - 48 and (code == 255).sum() == 48
+
+
+@pytest.mark.parametrize("dims,nv,prec,dots", [((128, 128, 128), 32, 0, False),
+                                               ((128, 128, 128), 32, 0, True),
+                                               ((13, 11, 9), 8, 0, False),
+                                               ((20, 7, 31), 1, 1, True),
+                                               ((64, 40, 3), 64, 0, True)])
+def test_grid_schedule_covers_every_row_once(X, dims, nv, prec, dots):
+    nx, ny, nz = dims
+    g, ipw, items = X.grid_schedule(nx, ny, nz, nv, prec, dots)
+    assert g % 8 == 0 and g * ipw >= len(items)
+    if dots:
+        assert g <= 1024
+    w = 16 // (8 if prec == 0 else 4)
+    rpb = 256 // (nv // min(w, nv))
+    tz = 16 if rpb >= 256 else 8 if rpb >= 64 else 4 if rpb >= 16 else 2 if rpb >= 4 else 1
+    ty = rpb // tz
+    hit = np.zeros((nz, ny, nx), np.int32)
+    for y0, z0, c0, c1 in items:
+        assert 0 <= c0 < c1 <= nx
+        hit[z0:z0 + tz, y0:y0 + ty, c0:c1] += 1
+    assert (hit == 1).all()
