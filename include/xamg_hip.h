@@ -134,7 +134,10 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
                      int64_t *grid, int64_t *ntiles, int64_t *tpw, int64_t *rows);
 /* Launch-shape knobs: "spmv_strip_budget" (bytes, default 8 MiB), "spmv_tile_passes"
  * (default 8), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
- * when 2*fold*nv*sizeof(value) exceeds it). */
+ * when 2*fold*nv*sizeof(value) exceeds it), "spmv_grid_enable" (0/1, default 0: the
+ * structured-grid SpMV of matrices with a grid view), "spmv_grid_items" (default 2048),
+ * "spmv_grid_min_seg" (default 32), "unet_fuse_up" (0/1, default 1: U-Net inference reads
+ * the decoder's upsampled inputs through the convolution instead of writing them). */
 int xh_set_option(const char *key, int64_t value);
 
 /* Compressed block: the reference's compact-index CSR block, as csr_matrix::compress

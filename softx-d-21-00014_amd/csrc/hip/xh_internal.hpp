@@ -144,6 +144,10 @@ struct Tuning {
     int grid_min_seg = 32;          // shortest x segment of a work item
 };
 extern Tuning g_tune;
+namespace unet {
+void set_fuse_up(int on); // 0: U-Net inference materialises the upsampled tensors
+}
+
 
 // Scalar state of a Krylov solve, resident in device memory.  One slot per RHS.
 struct SolverState {

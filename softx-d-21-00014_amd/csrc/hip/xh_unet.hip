@@ -75,8 +75,11 @@ __device__ __forceinline__ int64_t neigh(const Dims &g, int64_t m, int t) {
 // ---- convolution forward / data gradient: implicit GEMM on MFMA ----
 // out[m][co] = act(sum_k A[m][k] B[k][co] + bias[co] + add[m][co]); A gathered from `in`
 // (cin channels, k = t*cin + ci), B = wt[co][k] (kpad >= 27*cin, zero padded).
-// 4 waves x 16 voxels per workgroup; each wave all cout/16 channel tiles.
-template <int CT>
+// 4 waves per workgroup; a wave computes MT tiles of 16 voxels for all cout/16 channel
+// tiles, so every B fragment feeds MT MFMAs.  UP: `in` is the coarse volume of a nearest
+// x2 upsampling (d/2 x h/2 x w/2) and the fine neighbour (z, y, x) reads coarse voxel
+// (z/2, y/2, x/2): the upsampled tensor is never written.
+template <int CT, int MT, bool UP>
 __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__ in, int cin,
                                               const bf16_t *__restrict__ wt, int kpad,
                                               const float *__restrict__ bias,
@@ -84,82 +87,106 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
                                               bf16_t *__restrict__ out) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t M = g.vox();
-    const int64_t m0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    const int64_t m0 = ((int64_t)blockIdx.x * 4 + wave) * 16 * MT;
     if (m0 >= M)
         return;
     const int cout = CT * 16;
     const int K = 27 * cin;
     const int r = lane & 15, kg = lane >> 4;
-    const int64_t m = m0 + r;
-    const bool mok = m < M;
-    // the lane's voxel, decoded once
-    const int vx = (int)(m % g.w), vy = (int)((m / g.w) % g.h),
-              vz = (int)((m / ((int64_t)g.w * g.h)) % g.d);
-    f32x4 acc[CT];
+    // the lane's voxel of each tile, decoded once
+    int vx[MT], vy[MT], vz[MT], vb[MT];
+    bool mok[MT];
 #pragma unroll
-    for (int c = 0; c < CT; ++c)
-        acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < MT; ++t) {
+        const int64_t m = m0 + 16 * t + r;
+        mok[t] = m < M;
+        vx[t] = (int)(m % g.w);
+        vy[t] = (int)((m / g.w) % g.h);
+        const int64_t pl = m / ((int64_t)g.w * g.h);
+        vz[t] = (int)(pl % g.d);
+        vb[t] = (int)(pl / g.d);
+    }
+    f32x4 acc[MT][CT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+            acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
     // tap and channel of the lane's first k, advanced by 32 per step (cin % 8 == 0)
     int tt = (8 * kg) / cin, cc = 8 * kg - tt * cin;
     for (int kb = 0; kb < kpad; kb += 32) {
         const int k0 = kb + 8 * kg;
-        bf16x8 a;
+        bf16x8 a[MT];
         if ((cin & 7) == 0) {
             // 8 consecutive channels of one tap
-            uint4 q = make_uint4(0, 0, 0, 0);
-            if (mok && k0 < K) {
-                const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
-                const int zz = vz + dz, yy = vy + dy, xx = vx + dx;
-                if (zz >= 0 && zz < g.d && yy >= 0 && yy < g.h && xx >= 0 && xx < g.w) {
-                    const int64_t nb = m + ((int64_t)dz * g.h + dy) * g.w + dx;
+            const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                uint4 q = make_uint4(0, 0, 0, 0);
+                const int zz = vz[t] + dz, yy = vy[t] + dy, xx = vx[t] + dx;
+                if (mok[t] && k0 < K && zz >= 0 && zz < g.d && yy >= 0 && yy < g.h && xx >= 0 &&
+                    xx < g.w) {
+                    int64_t nb;
+                    if constexpr (UP)
+                        nb = (((int64_t)vb[t] * (g.d / 2) + zz / 2) * (g.h / 2) + yy / 2) *
+                                 (g.w / 2) + xx / 2;
+                    else
+                        nb = m0 + 16 * t + r + ((int64_t)dz * g.h + dy) * g.w + dx;
                     q = *reinterpret_cast<const uint4 *>(in + nb * cin + cc);
                 }
+                a[t] = __builtin_bit_cast(bf16x8, q);
             }
-            a = __builtin_bit_cast(bf16x8, q);
             cc += 32;
             while (cc >= cin) {
                 cc -= cin;
                 ++tt;
             }
         } else {
-            bf16_t e[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = k0 + j;
-                bf16_t v = 0;
-                if (mok && k < K) {
-                    const int t = k / cin, ci = k - t * cin;
-                    const int64_t nb = neigh(g, m, t);
-                    if (nb >= 0)
-                        v = in[nb * cin + ci];
+            for (int t = 0; t < MT; ++t) {
+                bf16_t e[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = k0 + j;
+                    bf16_t v = 0;
+                    if (mok[t] && k < K) {
+                        const int tp = k / cin, ci = k - tp * cin;
+                        const int64_t nb = neigh(g, m0 + 16 * t + r, tp);
+                        if (nb >= 0)
+                            v = in[nb * cin + ci];
+                    }
+                    e[j] = v;
                 }
-                e[j] = v;
+                a[t] = pack8(e);
             }
-            a = pack8(e);
         }
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const uint4 qb = *reinterpret_cast<const uint4 *>(wt + (size_t)(c * 16 + r) * kpad + k0);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, qb),
-                                                             acc[c], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a[t], __builtin_bit_cast(bf16x8, qb), acc[t][c], 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        const int co = c * 16 + r;
+    for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t mo = m0 + 4 * kg + i;
-            if (mo >= M)
-                continue;
-            float v = acc[c][i] + (bias ? bias[co] : 0.f);
-            if (add)
-                v = v + bf2f(add[mo * cout + co]);
-            if (relu)
-                v = v > 0.f ? v : 0.f;
-            out[mo * cout + co] = f2bf(v);
+        for (int c = 0; c < CT; ++c) {
+            const int co = c * 16 + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t mo = m0 + 16 * t + 4 * kg + i;
+                if (mo >= M)
+                    continue;
+                float v = acc[t][c][i] + (bias ? bias[co] : 0.f);
+                if (add)
+                    v = v + bf2f(add[mo * cout + co]);
+                if (relu)
+                    v = v > 0.f ? v : 0.f;
+                out[mo * cout + co] = f2bf(v);
+            }
         }
-    }
 }
 
 // ---- weight gradient: per (voxel chunk, tap) workgroup, LDS-staged MFMA ----
@@ -732,40 +759,57 @@ int get_work(xh_unet_s *U, int n, int d, int h, int w, bool grads, Work **out) {
     return 0;
 }
 
+// MT tiles of 16 voxels per wave: CT * MT = 4 accumulator tiles
+template <int CT, bool UP = false>
+void launch_conv(const Dims &g, const bf16_t *in, int cin, const bf16_t *wt, int kpad,
+                 const float *bias, const bf16_t *add, int relu, bf16_t *out, hipStream_t s) {
+    constexpr int MT = CT >= 3 ? 1 : 4 / CT;
+    const dim3 grid((unsigned)((g.vox() + 64 * MT - 1) / (64 * MT))), blk(256);
+    hipLaunchKernelGGL((k_conv<CT, MT, UP>), grid, blk, 0, s, g, in, cin, wt, kpad, bias, add,
+                       relu, out);
+}
+
+// up: `in` is the coarse input of a nearest x2 upsampling onto g (fused)
 void conv_fwd(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *in, const bf16_t *add,
-              bool relu, bf16_t *out) {
-    const dim3 grid((unsigned)((g.vox() + 63) / 64)), blk(256);
+              bool relu, bf16_t *out, bool up = false) {
+#define XH_CONV_CASE(CTV)                                                                      \
+    if (up)                                                                                    \
+        launch_conv<CTV, true>(g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out,          \
+                               U->stream);                                                     \
+    else                                                                                       \
+        launch_conv<CTV, false>(g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out,         \
+                                U->stream);
     switch (c.cout / 16) {
     case 1:
-        hipLaunchKernelGGL(k_conv<1>, grid, blk, 0, U->stream, g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out);
+        XH_CONV_CASE(1)
         break;
     case 2:
-        hipLaunchKernelGGL(k_conv<2>, grid, blk, 0, U->stream, g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out);
+        XH_CONV_CASE(2)
         break;
     case 3:
-        hipLaunchKernelGGL(k_conv<3>, grid, blk, 0, U->stream, g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out);
+        XH_CONV_CASE(3)
         break;
     default:
-        hipLaunchKernelGGL(k_conv<4>, grid, blk, 0, U->stream, g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out);
+        XH_CONV_CASE(4)
         break;
     }
+#undef XH_CONV_CASE
 }
 
 // gin = data gradient of conv c (c.cin must be a multiple of 16)
 void conv_data(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *gout, bf16_t *gin) {
-    const dim3 grid((unsigned)((g.vox() + 63) / 64)), blk(256);
     switch (c.cin / 16) {
     case 1:
-        hipLaunchKernelGGL(k_conv<1>, grid, blk, 0, U->stream, g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin);
+        launch_conv<1>(g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin, U->stream);
         break;
     case 2:
-        hipLaunchKernelGGL(k_conv<2>, grid, blk, 0, U->stream, g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin);
+        launch_conv<2>(g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin, U->stream);
         break;
     case 3:
-        hipLaunchKernelGGL(k_conv<3>, grid, blk, 0, U->stream, g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin);
+        launch_conv<3>(g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin, U->stream);
         break;
     default:
-        hipLaunchKernelGGL(k_conv<4>, grid, blk, 0, U->stream, g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin);
+        launch_conv<4>(g, gout, c.cout, c.wtf, c.kpadf, nullptr, nullptr, 0, gin, U->stream);
         break;
     }
 }
@@ -792,7 +836,12 @@ void relu_mask(xh_unet_s *U, int64_t n, const bf16_t *act, bf16_t *g) {
     hipLaunchKernelGGL(k_relu_mask, dim3(grid1(n)), dim3(256), 0, U->stream, n, act, g);
 }
 
-void forward(xh_unet_s *U, Work &W) {
+// 0: inference materialises the upsampled decoder inputs as training does
+int g_fuse_up = 1;
+
+// infer: the upsampled decoder inputs are not kept (no backward follows): the decoder's
+// first convolution reads the coarse tensor through the fused upsampling
+void forward(xh_unet_s *U, Work &W, bool infer = false) {
     const int L = U->L;
     hipStream_t st = U->stream;
     for (int l = 0; l < L; ++l) {
@@ -807,9 +856,14 @@ void forward(xh_unet_s *U, Work &W) {
     for (int l = L - 2; l >= 0; --l) {
         const Dims &g = W.g[l];
         const bf16_t *hup = l + 1 == L - 1 ? W.s[L - 1] : W.hh[l + 1];
-        const int64_t n = g.vox() * U->ch[l + 1];
-        hipLaunchKernelGGL(k_up, dim3(grid1(n)), dim3(256), 0, st, g, U->ch[l + 1], hup, W.u[l]);
-        conv_fwd(U, U->dec[2 * l], g, W.u[l], W.s[l], true, W.dd[l]);
+        if (infer && g_fuse_up) {
+            conv_fwd(U, U->dec[2 * l], g, hup, W.s[l], true, W.dd[l], true);
+        } else {
+            const int64_t n = g.vox() * U->ch[l + 1];
+            hipLaunchKernelGGL(k_up, dim3(grid1(n)), dim3(256), 0, st, g, U->ch[l + 1], hup,
+                               W.u[l]);
+            conv_fwd(U, U->dec[2 * l], g, W.u[l], W.s[l], true, W.dd[l]);
+        }
         conv_fwd(U, U->dec[2 * l + 1], g, W.dd[l], nullptr, true, W.hh[l]);
     }
     const bf16_t *h0 = L == 1 ? W.s[0] : W.hh[0];
@@ -934,6 +988,9 @@ void free_unet(xh_unet_s *U) {
 }
 
 } // namespace
+
+void set_fuse_up(int on) { g_fuse_up = on ? 1 : 0; }
+
 } // namespace unet
 } // namespace xh
 
@@ -1189,7 +1246,7 @@ extern "C" int xh_unet_infer(xh_unet U, int nx, int ny, int nz, int nv, const do
     column_rms(U, V, nv, db);
     hipLaunchKernelGGL(k_gather_in, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, db,
                        U->scale, W->in[0], nullptr, nullptr);
-    forward(U, *W);
+    forward(U, *W, true);
     // x0 = s * o * y
     hipLaunchKernelGGL(k_scale_out, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, W->y,
                        U->scale, U->oscale, dx0);

@@ -107,3 +107,22 @@ def test_training_and_solver_from_learned_x0(gpu):
     assert it["learned"] <= it["zero"], it
     # both starts reach the same solution to the solver's tolerance
     assert (np.abs(sol["learned"] - xs).max(0) / np.abs(xs).max(0)).max() < 1e-6
+
+
+def test_fused_upsampling_inference_is_identical(gpu):
+    # inference reads the decoder's upsampled inputs through the convolution (no
+    # upsampled tensor is written); the arithmetic is that of the materialised path
+    X = gpu
+    net = X.UNet((16, 32, 48), seed=3)
+    net.output_scale = 0.5
+    for n, nv in ((16, 3), (64, 2)):
+        b = X.DeviceArray.from_host(np.random.default_rng(n).standard_normal(n ** 3 * nv))
+        outs = []
+        for fuse in (0, 1):
+            X.set_option("unet_fuse_up", fuse)
+            x = X.DeviceArray(n ** 3 * nv)
+            net.infer(n, n, n, nv, b, x)
+            outs.append(x.to_host())
+        X.set_option("unet_fuse_up", 1)
+        assert np.isfinite(outs[0]).all() and np.abs(outs[0]).max() > 0
+        assert np.array_equal(outs[0], outs[1])
