@@ -116,3 +116,57 @@ def test_c_api_example_flow(gpu, O):
     # a wrong nv is an error code, not an abort
     assert h.XAMG_solver_solve1(nv + 1, sid) < 0
     assert h.XAMG_solver_destroy(sid) == 0
+
+
+@pytest.mark.parametrize("method", ["PCG", "RBiCGStab", "MergedRBiCGStab"])
+def test_c_api_system_file_and_registry_methods(gpu, tmp_path, method):
+    # a system file in the reference's reader layout (written here, read back per rank)
+    # feeds the C API; the solver role names one of the registry's other Krylov methods
+    X = gpu
+    h = api(X)
+    assert h.XAMG_init(0, None, b"") == 0
+    N, nv = 16, 2
+    row, col, val = X.cube(N)
+    n = N ** 3
+    b = X.rhs_uniform(11, n, nv)
+    path = str(tmp_path / "cube16.bin")
+    X.write_system(path, row, col, val, nv, b)
+    sysf = X.read_system(path, nv)
+    assert sysf["found"] == 1 and (sysf["x"] == 0).all()
+    cid = new(h, "XAMG_csr_matrix_create")
+    assert h.XAMG_csr_matrix_alloc(n, len(sysf["col"]), cid) == 0
+    r32, c32 = sysf["row"].astype(np.int32), sysf["col"].astype(np.int32)
+    v = np.ascontiguousarray(sysf["val"])
+    assert h.XAMG_csr_matrix_fill(r32.ctypes.data, c32.ctypes.data, v.ctypes.data, cid) == 0
+    mid = new(h, "XAMG_matrix_create")
+    assert h.XAMG_matrix_construct(cid, mid) == 0
+    pid = new(h, "XAMG_param_create")
+    for role, key, val_s in ((b"solver", b"method", method.encode()),
+                             (b"preconditioner", b"method", b"MultiGrid"),
+                             (b"pre_smoother", b"method", b"Jacobi"),
+                             (b"post_smoother", b"method", b"Jacobi")):
+        assert h.XAMG_param_add_value_s(role, key, val_s, pid) == 0
+    assert h.XAMG_param_add_value_d(b"solver", b"rel_tolerance", 1e-9, pid) == 0
+    assert h.XAMG_param_add_value_d(b"solver", b"abs_tolerance", 0.0, pid) == 0
+    h.XAMG_param_set_default(pid)
+    xid = new(h, "XAMG_vector_create", 2)
+    bid = new(h, "XAMG_vector_create", 2)
+    assert h.XAMG_vector_alloc_d(n, nv, xid) == 0 and h.XAMG_vector_alloc_d(n, nv, bid) == 0
+    bb = np.ascontiguousarray(sysf["b"])
+    xx = np.ascontiguousarray(sysf["x"])
+    assert h.XAMG_vector_upload_d(bb.ctypes.data, n, 0, nv, bid) == 0
+    assert h.XAMG_vector_upload_d(xx.ctypes.data, n, 0, nv, xid) == 0
+    sid = S()
+    assert h.XAMG_solver_create2(mid, pid, xid, bid, C.byref(sid)) == 0, \
+        h.XAMG_get_last_error()
+    sid = sid.value
+    assert h.XAMG_solver_solve1(nv, sid) == 0
+    conv = np.zeros(nv, np.int32)
+    assert h.XAMG_solver_get_convergence_info(conv.ctypes.data, nv, sid) == 0
+    assert conv.all()
+    x = np.zeros(n * nv)
+    assert h.XAMG_vector_download_d(x.ctypes.data, n, 0, nv, xid) == 0
+    A = sp.csr_matrix((val.copy(), col.astype(np.int64), row.astype(np.int64)), shape=(n, n))
+    xs = spla.spsolve(A.tocsc(), b.reshape(-1, nv))
+    assert (np.abs(x.reshape(-1, nv) - xs).max(0) / np.abs(xs).max(0)).max() < 1e-7
+    assert h.XAMG_solver_destroy(sid) == 0
