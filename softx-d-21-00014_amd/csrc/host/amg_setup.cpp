@@ -651,18 +651,35 @@ void estimate_eigs(const CsrHost &A, int iters, uint64_t seed, double &mn, doubl
         uint64_t z = splitmix64(seed * 0x100000001B3ull + (uint64_t)i);
         x[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
     }
+    // dots as fixed-size chunk partials summed in chunk order: the estimate (and with it
+    // every Chebyshev coefficient) is the same whatever the thread count and schedule
+    constexpr int64_t kChunk = 4096;
+    const int64_t nchunk = (n + kChunk - 1) / kChunk;
+    std::vector<double> pxy(nchunk), pxx(nchunk), pyy(nchunk);
     double lam = 0.0;
     for (int it = 0; it < std::max(1, iters); ++it) {
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < nchunk; ++c) {
+            double cxy = 0.0, cxx = 0.0, cyy = 0.0;
+            const int64_t i1 = std::min(n, (c + 1) * kChunk);
+            for (int64_t i = c * kChunk; i < i1; ++i) {
+                double s = 0.0;
+                for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+                    s += A.val[j] * isd[A.col[j]] * x[A.col[j]];
+                y[i] = isd[i] * s;
+                cxy += x[i] * y[i];
+                cxx += x[i] * x[i];
+                cyy += y[i] * y[i];
+            }
+            pxy[c] = cxy;
+            pxx[c] = cxx;
+            pyy[c] = cyy;
+        }
         double xy = 0.0, xx = 0.0, yy = 0.0;
-#pragma omp parallel for reduction(+ : xy, xx, yy) schedule(static)
-        for (int64_t i = 0; i < n; ++i) {
-            double s = 0.0;
-            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
-                s += A.val[j] * isd[A.col[j]] * x[A.col[j]];
-            y[i] = isd[i] * s;
-            xy += x[i] * y[i];
-            xx += x[i] * x[i];
-            yy += y[i] * y[i];
+        for (int64_t c = 0; c < nchunk; ++c) {
+            xy += pxy[c];
+            xx += pxx[c];
+            yy += pyy[c];
         }
         lam = xx > 0 ? xy / xx : 0.0;
         double nrm = std::sqrt(yy);
