@@ -375,6 +375,25 @@ __global__ void k_head(int64_t M, int C, const bf16_t *__restrict__ h, const flo
     }
 }
 
+// inference head fused with the output scaling: x0[v*nv + k] = s_k * o * (w . h[k*V + v] + b),
+// threads ordered by (v, k) with k fastest so x0 is written contiguously (each thread
+// reads its voxel's C channels, one 16-byte-aligned run)
+__global__ void k_head_out(int64_t V, int nv, int C, const bf16_t *__restrict__ h,
+                           const float *__restrict__ w, const float *__restrict__ b,
+                           const float *__restrict__ scale, float o, double *__restrict__ x0) {
+    const int64_t n = V * nv;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i % nv);
+        const int64_t v = i / nv, m = (int64_t)k * V + v;
+        float s = 0.f;
+        for (int c = 0; c < C; ++c)
+            s = s + w[c] * bf2f(h[m * C + c]);
+        const float y = s + b[0];
+        x0[i] = (double)scale[k] * (double)o * (double)y;
+    }
+}
+
 // head gradients: gh[m][c] = gy[m] w[c]; per-chunk partials of gw[c] = sum gy h, gb = sum gy
 __global__ void k_head_bwd(int64_t M, int C, const bf16_t *__restrict__ h, const float *__restrict__ w,
                            const float *__restrict__ gy, bf16_t *__restrict__ gh, int64_t chunk,
@@ -496,17 +515,6 @@ __global__ void k_gather_in(int64_t V, int nv, const double *__restrict__ b,
     }
 }
 
-// x0[vox][k] = scale[k] * o * y[k][vox]
-__global__ void k_scale_out(int64_t V, int nv, const float *__restrict__ y,
-                            const float *__restrict__ scale, float o, double *__restrict__ x0) {
-    const int64_t n = V * nv;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / V);
-        const int64_t v = i % V;
-        x0[v * nv + k] = (double)scale[k] * (double)o * (double)y[i];
-    }
-}
 
 // a[i] *= f
 __global__ void k_scale_f32(int64_t n, float *__restrict__ a, float f) {
@@ -866,6 +874,8 @@ void forward(xh_unet_s *U, Work &W, bool infer = false) {
         }
         conv_fwd(U, U->dec[2 * l + 1], g, W.dd[l], nullptr, true, W.hh[l]);
     }
+    if (infer)
+        return; // the head runs fused with the output scaling (k_head_out)
     const bf16_t *h0 = L == 1 ? W.s[0] : W.hh[0];
     const int64_t V0 = W.g[0].vox();
     hipLaunchKernelGGL(k_head, dim3(grid1(V0)), dim3(256), 0, st, V0, U->hc, h0, U->hw, U->hb, W.y);
@@ -1247,9 +1257,10 @@ extern "C" int xh_unet_infer(xh_unet U, int nx, int ny, int nz, int nv, const do
     hipLaunchKernelGGL(k_gather_in, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, db,
                        U->scale, W->in[0], nullptr, nullptr);
     forward(U, *W, true);
-    // x0 = s * o * y
-    hipLaunchKernelGGL(k_scale_out, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, W->y,
-                       U->scale, U->oscale, dx0);
+    // x0 = s * o * (head of the top decoder level)
+    const bf16_t *h0 = U->L == 1 ? W->s[0] : W->hh[0];
+    hipLaunchKernelGGL(k_head_out, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, U->hc,
+                       h0, U->hw, U->hb, U->scale, U->oscale, dx0);
     UN_HIP(hipGetLastError());
     UN_HIP(hipStreamSynchronize(U->stream));
     return 0;
