@@ -65,3 +65,35 @@ def test_setup_is_deterministic(X):
             if k in da:
                 for u, v in zip(da[k][2:], db[k][2:]):
                     assert np.array_equal(u, v)
+
+
+def test_setup_does_not_depend_on_the_thread_count(tmp_path):
+    # the hierarchy (matrices and the Chebyshev bounds from the power iteration) is the
+    # same bit for bit with 1 and 7 OpenMP threads
+    import json
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = (
+        "import sys, json, hashlib; sys.path.insert(0, %r)\n"
+        "import numpy as np, pyxamg as X\n"
+        "r, c, v, _ = X.varcoef7(18, 14, 11, sigma=1.0, seed=3)\n"
+        "lv = X.Amg(r, c, v).export()\n"
+        "h = hashlib.sha256()\n"
+        "for d in lv:\n"
+        "    for key in ('A', 'P', 'R'):\n"
+        "        if key in d:\n"
+        "            for a in d[key][2:]:\n"
+        "                h.update(np.ascontiguousarray(a).tobytes())\n"
+        "    h.update(np.array(d['eig'], np.float64).tobytes())\n"
+        "print(json.dumps(dict(h=h.hexdigest(), eig=[list(map(float, d['eig'])) for d in lv])))\n"
+        % os.path.join(here, "..", "softx-d-21-00014_amd"))
+    out = []
+    for nt in ("1", "7"):
+        env = dict(os.environ, OMP_NUM_THREADS=nt)
+        p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert p.returncode == 0, p.stderr
+        out.append(json.loads(p.stdout.strip().splitlines()[-1]))
+    assert out[0]["h"] == out[1]["h"], (out[0]["eig"], out[1]["eig"])
