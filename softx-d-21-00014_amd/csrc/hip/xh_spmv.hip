@@ -902,10 +902,7 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
-    // The Chebyshev steps stay on the row-parallel kernel: their grid instantiation (x
-    // update through a runtime-selected pointer) does not compile with this toolchain.
-    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
-                  OP != OP_CHEB0 && OP != OP_CHEB) {
+    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32) {
         if (A.gcode != nullptr && g_tune.grid_enable && A.row_ind == nullptr) {
             const GridLaunch G = grid_plan(A.gx, A.gy, A.gz, RPB, cap);
             hipLaunchKernelGGL((k_spmv_grid<VT, IT, NV, OP>), dim3(G.grid), dim3(kBlock), 0, s,

@@ -60,6 +60,8 @@ CASES = [
     ("rbicgstab_hsgs_nograph", 20, 4, "RBiCGStab", "HSGS", 1, {"graph": 0}),
     ("merged_rbicgstab_jacobi", 20, 4, "MergedRBiCGStab", "Jacobi", 2, {}),
     ("merged_rbicgstab_nv64", 16, 64, "MergedRBiCGStab", "Chebyshev", 3, {}),
+    ("pcg_fp32_precond", 20, 8, "PCG", "Chebyshev", 3, {"mg_prec": 2}),
+    ("merged_rbicgstab_wcycle", 16, 4, "MergedRBiCGStab", "Chebyshev", 2, {"mg_cycle": 1}),
 ]
 
 
@@ -80,9 +82,13 @@ def test_solver_parity(gpu, O, case):
     if g["iters"] == r["iters"]:
         # same iteration path: the per-check residual histories agree to rounding level
         # (the recurrence residuals theta - omega*phi and eta - omega*theta are differences
-        # of nearly equal terms near convergence: rounding relative to the initial norm)
-        np.testing.assert_allclose(g["hist"], r["hist"], rtol=1e-8,
-                                   atol=1e-11 * r["hist"][0].max())
+        # of nearly equal terms near convergence: rounding relative to the initial norm).
+        # With an fp32 preconditioner the fp64 residual is rounded to fp32 before each
+        # V-cycle, so a last-bit difference from the dot order can flip one fp32 rounding:
+        # the histories then agree to the fp32 scale.
+        fp32 = extra.get("mg_prec", 0) > 0
+        np.testing.assert_allclose(g["hist"], r["hist"], rtol=1e-5 if fp32 else 1e-8,
+                                   atol=(1e-8 if fp32 else 1e-11) * r["hist"][0].max())
 
 
 @pytest.mark.parametrize("method", ["BiCGStab", "MergedBiCGStab"])
