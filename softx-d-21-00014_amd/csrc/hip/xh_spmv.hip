@@ -329,7 +329,7 @@ __device__ __forceinline__ void epi_load(int64_t i, size_t o, size_t oy, int k0,
 // `in` holds the operands epi_load read for the row.
 template <typename VT, int NV, int W, int OP, int NP>
 __device__ __forceinline__ void row_epilogue(
-    int64_t i, size_t o, size_t oy, int k0, const VT (&acc)[W], double (&part)[NP][W],
+    size_t o, size_t oy, const VT (&acc)[W], double (&part)[NP][W],
     const EpiIn<VT, W> &in, void *__restrict__ yv, VT *__restrict__ out2,
     VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last, int x_zero, int out_f32) {
     using P = Pack<VT, W>;
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         }
         EpiIn<VT, W> in;
         epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last, x_zero);
-        row_epilogue<VT, NV, W, OP>(i, o, oy, k0, acc, part, in, yv, out2, out3, xupdv, c, last,
+        row_epilogue<VT, NV, W, OP>(o, oy, acc, part, in, yv, out2, out3, xupdv, c, last,
                                     x_zero, out_f32);
         jb = nb;
         je = ne;
@@ -587,9 +587,8 @@ struct GChunk {
     VT v[E];
 };
 
-template <typename VT, typename IT, int NV, int W>
-__device__ __forceinline__ void load_gchunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
-                                            const VT *__restrict__ val,
+template <typename VT, int NV, int W>
+__device__ __forceinline__ void load_gchunk(uint32_t j, uint32_t e, const VT *__restrict__ val,
                                             const uint8_t *__restrict__ gcode,
                                             GChunk<VT, NV, W> &ch) {
     // every lane loads, at an index clamped into the row (or to entry 0 for an empty
@@ -629,7 +628,7 @@ __device__ __forceinline__ Pack<VT, W> lds_row(const lds_t<Pack<VT, W>> *src) {
 // window reads go in batches of 4, unconditionally (a slot past the row end holds the
 // row's last code, a valid window address), so a batch is in flight together; only the
 // accumulation of a slot past the row end is skipped.
-template <typename VT, typename IT, int NV, int W, int NLY, int NL, int L>
+template <typename VT, int NV, int W, int NLY, int NL, int L>
 __device__ __forceinline__ void consume_gchunk(uint32_t j, uint32_t e, const GChunk<VT, NV, W> &ch,
                                                int pos, int lbase,
                                                const lds_t<Pack<VT, W>> *win, int lane,
@@ -669,11 +668,10 @@ __device__ __forceinline__ void consume_gchunk(uint32_t j, uint32_t e, const GCh
     });
 }
 
-template <typename VT, typename IT, int NV, int OP>
+template <typename VT, int NV, int OP>
 __global__ __launch_bounds__(kBlock) void k_spmv_grid(
-    const uint32_t *__restrict__ row, const IT *__restrict__ col, const VT *__restrict__ val,
-    const uint8_t *__restrict__ gcode, const uint32_t *__restrict__ pbase, int shift,
-    const GridSched g, const VT *__restrict__ x, void *__restrict__ yv, const VT *__restrict__ b,
+    const uint32_t *__restrict__ row, const VT *__restrict__ val,
+    const uint8_t *__restrict__ gcode, const GridSched g, const VT *__restrict__ x, void *__restrict__ yv, const VT *__restrict__ b,
     const VT *__restrict__ o1, const VT *__restrict__ isd, const VT *__restrict__ rr,
     VT *__restrict__ out2, VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last,
     int x_zero, int out_f32, double *__restrict__ partials, const int *__restrict__ done) {
@@ -760,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_grid(
             const bool rv = valid && cc < c1;
             S.jb = rv ? S.nb : 0u;
             S.je = rv ? S.ne : 0u;
-            load_gchunk<VT, IT, NV, W>(S.jb, S.je, col, val, gcode, S.ch);
+            load_gchunk<VT, NV, W>(S.jb, S.je, val, gcode, S.ch);
             S.nb = row[rb + (cc + 3 < g.nx ? cc + 3 : g.nx)];
             S.ne = row[rb + (cc + 4 < g.nx ? cc + 4 : g.nx)];
             const int64_t i = rb + (cc < g.nx ? cc : g.nx - 1);
@@ -779,16 +777,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_grid(
 #pragma unroll
             for (int w = 0; w < W; ++w)
                 acc[w] = OP == OP_ADD ? S.in.a.v[w] : VT(0);
-            consume_gchunk<VT, IT, NV, W, NLY, NL, L>(jb, je, S.ch, cc, lbase, wr, lane, acc);
+            consume_gchunk<VT, NV, W, NLY, NL, L>(jb, je, S.ch, cc, lbase, wr, lane, acc);
             if (je - jb > (uint32_t)C::CH)
                 for (uint32_t j = jb + C::CH; j < je; j += C::CH) {
                     C ech;
-                    load_gchunk<VT, IT, NV, W>(j, je, col, val, gcode, ech);
-                    consume_gchunk<VT, IT, NV, W, NLY, NL, L>(j, je, ech, cc, lbase, wr, lane,
+                    load_gchunk<VT, NV, W>(j, je, val, gcode, ech);
+                    consume_gchunk<VT, NV, W, NLY, NL, L>(j, je, ech, cc, lbase, wr, lane,
                                                                acc);
                 }
             if (valid)
-                row_epilogue<VT, NV, W, OP>(i, o, o, k0, acc, part, S.in, yv, out2, out3, xupdv,
+                row_epilogue<VT, NV, W, OP>(o, o, acc, part, S.in, yv, out2, out3, xupdv,
                                             c, last, x_zero, out_f32);
             if (cc + 2 <= c1)
                 stage_store(cc + 2, S.s);
@@ -905,9 +903,9 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32) {
         if (A.gcode != nullptr && g_tune.grid_enable && A.row_ind == nullptr) {
             const GridLaunch G = grid_plan(A.gx, A.gy, A.gz, RPB, cap);
-            hipLaunchKernelGGL((k_spmv_grid<VT, IT, NV, OP>), dim3(G.grid), dim3(kBlock), 0, s,
-                               A.row, (const IT *)A.col, (const VT *)A.val, A.gcode,
-                               A.panel_base, A.panel_shift, G.g, (const VT *)x, y,
+            hipLaunchKernelGGL((k_spmv_grid<VT, NV, OP>), dim3(G.grid), dim3(kBlock), 0, s,
+                               A.row, (const VT *)A.val, A.gcode, G.g,
+                               (const VT *)x, y,
                                (const VT *)a.b, (const VT *)a.o1, (const VT *)a.isd,
                                (const VT *)a.r, (VT *)a.out2, (VT *)a.out3, a.xupd, (VT)a.c,
                                a.last, a.x_zero, a.out_f32, a.partials, a.done);
