@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
     for (int kb = 0; kb < kpad; kb += 32) {
         const int k0 = kb + 8 * kg;
         bf16x8 a[MT];
-        if ((cin & 7) == 0) {
+        if (UP || (cin & 7) == 0) { // the fused upsampling is only used with cin % 8 == 0
             // 8 consecutive channels of one tap
             const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
 #pragma unroll
@@ -142,6 +142,8 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
                 ++tt;
             }
         } else {
+            // cin not a multiple of 8 (the single-channel input layer): element gathers,
+            // neighbours from the decoded voxel coordinates
 #pragma unroll
             for (int t = 0; t < MT; ++t) {
                 bf16_t e[8];
@@ -151,9 +153,13 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
                     bf16_t v = 0;
                     if (mok[t] && k < K) {
                         const int tp = k / cin, ci = k - tp * cin;
-                        const int64_t nb = neigh(g, m0 + 16 * t + r, tp);
-                        if (nb >= 0)
+                        const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+                        const int zz = vz[t] + dz, yy = vy[t] + dy, xx = vx[t] + dx;
+                        if (zz >= 0 && zz < g.d && yy >= 0 && yy < g.h && xx >= 0 && xx < g.w) {
+                            const int64_t nb =
+                                m0 + 16 * t + r + ((int64_t)dz * g.h + dy) * g.w + dx;
                             v = in[nb * cin + ci];
+                        }
                     }
                     e[j] = v;
                 }
