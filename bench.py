@@ -34,7 +34,9 @@ def learned_x0(X, args):
     """Learned initial guess: 128^3 variable-coefficient 7-point operator (k lognormal,
     sigma 1, fixed seed), RHS sums of 8-32 Gaussian bumps; a 4-level U-Net (16-64
     channels) trained on solver pairs, then 256 held-out RHS (nv = 32) solved from x0 = 0
-    and from the network's x0 by the same solver to the same tolerance."""
+    and from the network's x0 by the same solver to the same tolerance, once with the fp64
+    preconditioner and once with the fp32 one (the learned x0 plus the fp32 preconditioner
+    is timed against the plain fp64 solve from x0 = 0)."""
     N, nv = 128, 32
     V = N ** 3
     row, col, val, _ = X.varcoef7(N, sigma=1.0, seed=2024)
@@ -44,6 +46,12 @@ def learned_x0(X, args):
                            poly_order=4, sweeps=1, rel_tol=1e-8, abs_tol=0.0, max_iters=100,
                            graph=1)
     S = X.Solver(H, prm, nv)
+    # the same method with the fp32 preconditioner (hierarchy, smoothers and SpMM inside
+    # the V-cycle in fp32; Krylov vectors, dots and the coarsest solve in fp64)
+    pm = X.default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Chebyshev",
+                          poly_order=4, sweeps=1, rel_tol=1e-8, abs_tol=0.0, max_iters=100,
+                          mg_prec=2, coarse_fp64=1, graph=1)
+    Sm = X.Solver(H, pm, nv)
     sets = (args.learned_pairs + nv - 1) // nv
     g0 = time.perf_counter()
     pairs = []
@@ -60,6 +68,7 @@ def learned_x0(X, args):
     g2 = time.perf_counter()
     del pairs
     it0, it1, t0s, t1s, tinf, diffs = [], [], 0.0, 0.0, 0.0, []
+    it2, it3, t2s, t3s, diffs_m = [], [], 0.0, 0.0, []
     for q in range(256 // nv):
         db = X.rhs_bumps_device(50000 + q, N, N, N, nv)
         dx = X.DeviceArray(V * nv)
@@ -82,8 +91,28 @@ def learned_x0(X, args):
         t0s += b_ - a
         t1s += e - c
         tinf += d - c
+        # fp32 preconditioner from x0 = 0, then from the network's x0
+        dxm = X.DeviceArray(V * nv)
+        X.hip().xh_synchronize()
+        f = time.perf_counter()
+        Sm.solve_device(db, dxm)
+        X.hip().xh_synchronize()
+        g = time.perf_counter()
+        it2.append(Sm.stats()[0])
+        dxl = X.DeviceArray(V * nv)
+        h = time.perf_counter()
+        net.infer(N, N, N, nv, db, dxl)
+        Sm.solve_device(db, dxl)
+        X.hip().xh_synchronize()
+        k = time.perf_counter()
+        it3.append(Sm.stats()[0])
+        t2s += g - f
+        t3s += k - h
         x0_, x1_ = dx.to_host().reshape(-1, nv), dx0.to_host().reshape(-1, nv)
         diffs.append(float((np.abs(x1_ - x0_).max(0) / np.abs(x0_).max(0)).max()))
+        x3_ = dxl.to_host().reshape(-1, nv)
+        diffs_m.append(float((np.abs(x3_ - x0_).max(0) / np.abs(x0_).max(0)).max()))
+        del dxm, dxl
     nr = 256
     return {
         "workload": "128^3 7-point variable-coefficient operator (k lognormal sigma 1), RHS "
@@ -101,6 +130,12 @@ def learned_x0(X, args):
         "inference_ms_per_rhs": round(tinf * 1e3 / nr, 4),
         "time_to_tolerance_speedup": round(t0s / t1s, 3) if t1s > 0 else None,
         "max_rel_diff_learned_vs_zero_start": max(diffs),
+        "iters_x0_zero_fp32_precond": float(np.mean(it2)),
+        "iters_x0_learned_fp32_precond": float(np.mean(it3)),
+        "ms_per_rhs_x0_zero_fp32_precond": round(t2s * 1e3 / nr, 4),
+        "ms_per_rhs_x0_learned_fp32_precond_incl_inference": round(t3s * 1e3 / nr, 4),
+        "speedup_learned_fp32_precond_vs_plain_fp64": round(t0s / t3s, 3) if t3s > 0 else None,
+        "max_rel_diff_learned_fp32_precond_vs_plain_fp64": max(diffs_m),
     }
 
 

@@ -146,6 +146,7 @@ struct Tuning {
 extern Tuning g_tune;
 namespace unet {
 void set_fuse_up(int on); // 0: U-Net inference materialises the upsampled tensors
+void set_conv_tile(int on); // 0: no LDS-staged convolution tiles
 }
 
 

@@ -1539,6 +1539,8 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.grid_min_seg = (int)value;
     else if (k == "unet_fuse_up" && (value == 0 || value == 1))
         unet::set_fuse_up((int)value);
+    else if (k == "unet_conv_tile" && (value == 0 || value == 1))
+        unet::set_conv_tile((int)value);
 
     else
         return fail(XH_EINVAL, "xh_set_option: unknown key or value out of range: " + k);

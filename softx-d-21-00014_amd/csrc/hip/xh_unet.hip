@@ -195,6 +195,134 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
         }
 }
 
+// ---- forward convolution over LDS-staged tiles (cout = 16) ----
+// A workgroup computes kTZ x kTY lines of kTX voxels.  The input those voxels read (the
+// tile with a one-voxel halo, or with UP the coarse voxels under it) is staged in LDS once,
+// zero outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes)
+// by voxel, so the 16 rows of an A fragment are 256 contiguous bytes.  All 27 taps then
+// read their A fragments from LDS instead of gathering them through the L1 per tap.  The k
+// order, the fragments and the epilogue are those of k_conv: the output is bit-identical.
+// The launch requires w % kTX == 0, h % kTY == 0 and d % kTZ == 0.
+constexpr int kTX = 64, kTY = 4, kTZ = 2;
+
+template <int NCG, bool UP>
+__global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restrict__ in,
+                                                   const bf16_t *__restrict__ wt, int kpad,
+                                                   const float *__restrict__ bias,
+                                                   const bf16_t *__restrict__ add, int relu,
+                                                   bf16_t *__restrict__ out) {
+    constexpr int CIN = 8 * NCG;
+    constexpr int SZ = UP ? kTZ / 2 + 2 : kTZ + 2;
+    constexpr int SY = UP ? kTY / 2 + 2 : kTY + 2;
+    constexpr int SX = UP ? kTX / 2 + 2 : kTX + 2;
+    constexpr int NCH = SZ * SY * SX * NCG; // 16-byte chunks staged
+    constexpr int PER = (NCH + 255) / 256;
+    constexpr int XT = kTX / 16;              // 16-voxel tiles per line
+    constexpr int LPW = kTZ * kTY / 4;        // lines per wave
+    constexpr int NT = LPW * XT;              // tiles per wave
+    __shared__ uint4 sm[NCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntx = g.w / kTX, nty = g.h / kTY, ntz = g.d / kTZ;
+    int64_t bid = blockIdx.x;
+    const int tx = (int)(bid % ntx);
+    bid /= ntx;
+    const int ty = (int)(bid % nty);
+    bid /= nty;
+    const int tz = (int)(bid % ntz);
+    const int nb = (int)(bid / ntz);
+    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    // the staged volume (coarse with UP) and the tile's origin in it
+    const int sd = UP ? g.d / 2 : g.d, sh = UP ? g.h / 2 : g.h, sw = UP ? g.w / 2 : g.w;
+    const int oz = (UP ? z0 / 2 : z0) - 1, oy = (UP ? y0 / 2 : y0) - 1,
+              ox = (UP ? x0 / 2 : x0) - 1;
+    {
+        uint4 st[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int q = tid + 256 * j;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (q < NCH) {
+                const int line = q / (SX * NCG), rem = q - line * (SX * NCG);
+                const int sx = rem / NCG, cg = rem - sx * NCG;
+                const int zz = oz + line / SY, yy = oy + line % SY, xx = ox + sx;
+                if (zz >= 0 && zz < sd && yy >= 0 && yy < sh && xx >= 0 && xx < sw)
+                    v = *reinterpret_cast<const uint4 *>(
+                        in + ((((int64_t)nb * sd + zz) * sh + yy) * sw + xx) * CIN + 8 * cg);
+            }
+            st[j] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int q = tid + 256 * j;
+            if (q < NCH) {
+                const int line = q / (SX * NCG), rem = q - line * (SX * NCG);
+                const int sx = rem / NCG, cg = rem - sx * NCG;
+                sm[(line * NCG + cg) * SX + sx] = st[j];
+            }
+        }
+    }
+    __syncthreads();
+    const int r = lane & 15, kg = lane >> 4;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    constexpr int K = 27 * CIN;
+    // tap and channel of the lane's first k, advanced by 32 per step (as in k_conv)
+    int tt = (8 * kg) / CIN, cc = 8 * kg - tt * CIN;
+    for (int kb = 0; kb < kpad; kb += 32) {
+        const int k0 = kb + 8 * kg;
+        const uint4 qb = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + k0);
+        const bool kok = k0 < K;
+        const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+        const int cg = cc >> 3;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int li = wave * LPW + t / XT, xt = t % XT;
+            const int iz = li / kTY, iy = li % kTY;
+            int sz, sy, sx;
+            if constexpr (UP) {
+                // fine offset -1 maps to coarse -1 (staged zero), as the fine bounds test
+                sz = ((iz + dz) >> 1) + 1;
+                sy = ((iy + dy) >> 1) + 1;
+                sx = ((16 * xt + r + dx) >> 1) + 1;
+            } else {
+                sz = iz + dz + 1;
+                sy = iy + dy + 1;
+                sx = 16 * xt + r + dx + 1;
+            }
+            uint4 q = sm[kok ? ((sz * SY + sy) * NCG + cg) * SX + sx : 0];
+            if (!kok)
+                q = make_uint4(0, 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                __builtin_bit_cast(bf16x8, q), __builtin_bit_cast(bf16x8, qb), acc[t], 0, 0, 0);
+        }
+        cc += 32;
+        while (cc >= CIN) {
+            cc -= CIN;
+            ++tt;
+        }
+    }
+    const float bv = bias ? bias[r] : 0.f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int li = wave * LPW + t / XT, xt = t % XT;
+        const int iz = li / kTY, iy = li % kTY;
+        const int64_t mb =
+            (((int64_t)nb * g.d + z0 + iz) * g.h + y0 + iy) * g.w + x0 + 16 * xt + 4 * kg;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t mo = mb + i;
+            float v = acc[t][i] + bv;
+            if (add)
+                v = v + bf2f(add[mo * 16 + r]);
+            if (relu)
+                v = v > 0.f ? v : 0.f;
+            out[mo * 16 + r] = f2bf(v);
+        }
+    }
+}
+
 // ---- weight gradient: per (voxel chunk, tap) workgroup, LDS-staged MFMA ----
 // part[chunk][t][ci][co] = sum_{m in chunk} in[m + off_t][ci] * g[m][co]
 // (ci padded to a multiple of 16 with zero rows; cin <= 64, cout <= 64)
@@ -783,9 +911,33 @@ void launch_conv(const Dims &g, const bf16_t *in, int cin, const bf16_t *wt, int
                        relu, out);
 }
 
+// 0: every forward convolution runs as k_conv (no LDS-staged tiles)
+int g_conv_tile = 1;
+
+template <int NCG, bool UP>
+void launch_conv_tile(const Dims &g, const bf16_t *in, const bf16_t *wt, int kpad,
+                      const float *bias, const bf16_t *add, int relu, bf16_t *out,
+                      hipStream_t s) {
+    const dim3 grid((unsigned)(g.vox() / (kTX * kTY * kTZ))), blk(256);
+    hipLaunchKernelGGL((k_conv_tile<NCG, UP>), grid, blk, 0, s, g, in, wt, kpad, bias, add,
+                       relu, out);
+}
+
 // up: `in` is the coarse input of a nearest x2 upsampling onto g (fused)
 void conv_fwd(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *in, const bf16_t *add,
               bool relu, bf16_t *out, bool up = false) {
+    // 16 output channels on volumes the tile divides: LDS-staged tiles (16 input channels,
+    // or 16 / 32 through the fused upsampling, whose coarse tile is small)
+    if (g_conv_tile && c.cout == 16 && g.w % kTX == 0 && g.h % kTY == 0 && g.d % kTZ == 0 &&
+        (c.cin == 16 || (up && c.cin == 32))) {
+        if (c.cin == 16 && !up)
+            launch_conv_tile<2, false>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
+        else if (c.cin == 16)
+            launch_conv_tile<2, true>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
+        else
+            launch_conv_tile<4, true>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
+        return;
+    }
 #define XH_CONV_CASE(CTV)                                                                      \
     if (up)                                                                                    \
         launch_conv<CTV, true>(g, in, c.cin, c.wt, c.kpad, c.b, add, (int)relu, out,          \
@@ -1006,6 +1158,7 @@ void free_unet(xh_unet_s *U) {
 } // namespace
 
 void set_fuse_up(int on) { g_fuse_up = on ? 1 : 0; }
+void set_conv_tile(int on) { g_conv_tile = on ? 1 : 0; }
 
 } // namespace unet
 } // namespace xh

@@ -40,7 +40,10 @@ def layer_slices(channels):
 
 @pytest.mark.parametrize("channels,size,batch", [((16, 32, 48), 8, 2),
                                                  ((16, 32, 48, 64), 8, 1),
-                                                 ((16,), 6, 3)])
+                                                 ((16,), 6, 3),
+                                                 # 64^3: the 16-channel convolutions run
+                                                 # on LDS-staged tiles
+                                                 ((16, 32), 64, 1)])
 def test_loss_and_gradients_match_oracle(gpu, channels, size, batch):
     X = gpu
     net = X.UNet(channels, seed=7)
@@ -126,3 +129,30 @@ def test_fused_upsampling_inference_is_identical(gpu):
         X.set_option("unet_fuse_up", 1)
         assert np.isfinite(outs[0]).all() and np.abs(outs[0]).max() > 0
         assert np.array_equal(outs[0], outs[1])
+
+
+def test_tiled_convolution_is_identical(gpu):
+    # forward convolutions with 16 output channels on volumes with w % 64 == h % 4 ==
+    # d % 2 == 0 read their input from LDS-staged tiles (plain, and through the fused
+    # upsampling with 16 or 32 coarse channels); fragments, k order and epilogue are those
+    # of the per-tap gathers, so the result is bit-identical
+    X = gpu
+    try:
+        for ch, fuse in (((16, 32, 48), 1), ((16, 32, 48), 0), ((16, 16, 32), 1)):
+            net = X.UNet(ch, seed=5)
+            net.output_scale = 0.5
+            X.set_option("unet_fuse_up", fuse)
+            for (nx, ny, nz), nv in (((64, 64, 64), 2), ((128, 32, 16), 3)):
+                V = nx * ny * nz
+                b = X.DeviceArray.from_host(np.random.default_rng(nx + nv).standard_normal(V * nv))
+                outs = []
+                for tile in (0, 1):
+                    X.set_option("unet_conv_tile", tile)
+                    x = X.DeviceArray(V * nv)
+                    net.infer(nx, ny, nz, nv, b, x)
+                    outs.append(x.to_host())
+                assert np.isfinite(outs[0]).all() and np.abs(outs[0]).max() > 0
+                assert np.array_equal(outs[0], outs[1]), (ch, fuse, nx, ny, nz)
+    finally:
+        X.set_option("unet_conv_tile", 1)
+        X.set_option("unet_fuse_up", 1)
