@@ -200,10 +200,54 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
 // tile with a one-voxel halo, or with UP the coarse voxels under it) is staged in LDS once,
 // zero outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes)
 // by voxel, so the 16 rows of an A fragment are 256 contiguous bytes.  All 27 taps then
-// read their A fragments from LDS instead of gathering them through the L1 per tap.  The k
-// order, the fragments and the epilogue are those of k_conv: the output is bit-identical.
-// The launch requires w % kTX == 0, h % kTY == 0 and d % kTZ == 0.
+// read their input fragments from LDS instead of gathering them through the L1 per tap; the
+// weight fragments of all k steps are loaded while the tile is staged.  The weights are the
+// A operand (the MFMA gives the transposed output tile, 4 channels of a voxel per lane, so
+// the stores are 8-byte vectors); the products, their k order and the epilogue's arithmetic
+// are those of k_conv, so the output is bit-identical.  The launch requires w % kTX == 0,
+// h % kTY == 0 and d % kTZ == 0.
 constexpr int kTX = 64, kTY = 4, kTZ = 2;
+
+// Epilogue of the tiled convolutions (weights as the A operand): lane (r, kg) of tile t
+// holds channels 4kg..4kg+3 of voxel r, written (and `add` read) as 8 bytes.  Per element
+// act(acc + bias + add), as in k_conv.
+template <int NT, int XT, int LPW>
+__device__ __forceinline__ void conv_tile_store(const Dims &g, int nb, int z0, int y0, int x0,
+                                                int wave, int r, int kg, const f32x4 (&acc)[NT],
+                                                const float *__restrict__ bias,
+                                                const bf16_t *__restrict__ add, int relu,
+                                                bf16_t *__restrict__ out) {
+    float bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        bv[i] = bias ? bias[4 * kg + i] : 0.f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int li = wave * LPW + t / XT, xt = t % XT;
+        const int iz = li / kTY, iy = li % kTY;
+        const int64_t mo =
+            (((int64_t)nb * g.d + z0 + iz) * g.h + y0 + iy) * g.w + x0 + 16 * xt + r;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v[i] = acc[t][i] + bv[i];
+        if (add) {
+            const uint2 a = *reinterpret_cast<const uint2 *>(add + mo * 16 + 4 * kg);
+            v[0] = v[0] + bf2f((bf16_t)(a.x & 0xffffu));
+            v[1] = v[1] + bf2f((bf16_t)(a.x >> 16));
+            v[2] = v[2] + bf2f((bf16_t)(a.y & 0xffffu));
+            v[3] = v[3] + bf2f((bf16_t)(a.y >> 16));
+        }
+        if (relu)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                v[i] = v[i] > 0.f ? v[i] : 0.f;
+        uint2 o;
+        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2 *>(out + mo * 16 + 4 * kg) = o;
+    }
+}
 
 template <int NCG, bool UP>
 __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restrict__ in,
@@ -232,6 +276,13 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     const int nb = (int)(bid / ntz);
     const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
     // the staged volume (coarse with UP) and the tile's origin in it
+    // every k step's weight fragment, loaded while the tile is staged
+    constexpr int KS = (27 * CIN + 31) / 32; // kpad == 32 * KS
+    const int r = lane & 15, kg = lane >> 4;
+    uint4 qb[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+        qb[ks] = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + 32 * ks + 8 * kg);
     const int sd = UP ? g.d / 2 : g.d, sh = UP ? g.h / 2 : g.h, sw = UP ? g.w / 2 : g.w;
     const int oz = (UP ? z0 / 2 : z0) - 1, oy = (UP ? y0 / 2 : y0) - 1,
               ox = (UP ? x0 / 2 : x0) - 1;
@@ -262,7 +313,6 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
         }
     }
     __syncthreads();
-    const int r = lane & 15, kg = lane >> 4;
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -270,10 +320,9 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     constexpr int K = 27 * CIN;
     // tap and channel of the lane's first k, advanced by 32 per step (as in k_conv)
     int tt = (8 * kg) / CIN, cc = 8 * kg - tt * CIN;
-    for (int kb = 0; kb < kpad; kb += 32) {
-        const int k0 = kb + 8 * kg;
-        const uint4 qb = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + k0);
-        const bool kok = k0 < K;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const bool kok = 32 * ks + 8 * kg < K;
         const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
         const int cg = cc >> 3;
 #pragma unroll
@@ -294,8 +343,9 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
             uint4 q = sm[kok ? ((sz * SY + sy) * NCG + cg) * SX + sx : 0];
             if (!kok)
                 q = make_uint4(0, 0, 0, 0);
+            // weights as the A operand: D[co][voxel], each lane 4 channels of one voxel
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                __builtin_bit_cast(bf16x8, q), __builtin_bit_cast(bf16x8, qb), acc[t], 0, 0, 0);
+                __builtin_bit_cast(bf16x8, qb[ks]), __builtin_bit_cast(bf16x8, q), acc[t], 0, 0, 0);
         }
         cc += 32;
         while (cc >= CIN) {
@@ -303,24 +353,73 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
             ++tt;
         }
     }
-    const float bv = bias ? bias[r] : 0.f;
+    conv_tile_store<NT, XT, LPW>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, add, relu, out);
+}
+
+// The single-channel input layer (cin = 1, K = 27 taps in one k step of 32) over the same
+// tiles: the staged line holds one bf16 per voxel and each lane assembles its 8 taps of
+// the A fragment from LDS, as k_conv assembles them from element gathers.
+__global__ __launch_bounds__(256) void k_conv_tile1(Dims g, const bf16_t *__restrict__ in,
+                                                    const bf16_t *__restrict__ wt, int kpad,
+                                                    const float *__restrict__ bias, int relu,
+                                                    bf16_t *__restrict__ out) {
+    constexpr int SZ = kTZ + 2, SY = kTY + 2, SX = kTX + 2;
+    constexpr int NS = SZ * SY * SX;
+    constexpr int PER = (NS + 255) / 256;
+    constexpr int XT = kTX / 16, LPW = kTZ * kTY / 4, NT = LPW * XT;
+    __shared__ bf16_t sm[NS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntx = g.w / kTX, nty = g.h / kTY, ntz = g.d / kTZ;
+    int64_t bid = blockIdx.x;
+    const int tx = (int)(bid % ntx);
+    bid /= ntx;
+    const int ty = (int)(bid % nty);
+    bid /= nty;
+    const int tz = (int)(bid % ntz);
+    const int nb = (int)(bid / ntz);
+    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    {
+        bf16_t st[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int q = tid + 256 * j;
+            bf16_t v = 0;
+            if (q < NS) {
+                const int line = q / SX, sx = q - line * SX;
+                const int zz = z0 - 1 + line / SY, yy = y0 - 1 + line % SY, xx = x0 - 1 + sx;
+                if (zz >= 0 && zz < g.d && yy >= 0 && yy < g.h && xx >= 0 && xx < g.w)
+                    v = in[(((int64_t)nb * g.d + zz) * g.h + yy) * g.w + xx];
+            }
+            st[j] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int q = tid + 256 * j;
+            if (q < NS)
+                sm[q] = st[j];
+        }
+    }
+    __syncthreads();
+    const int r = lane & 15, kg = lane >> 4;
+    const uint4 qb = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + 8 * kg);
+    f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int li = wave * LPW + t / XT, xt = t % XT;
         const int iz = li / kTY, iy = li % kTY;
-        const int64_t mb =
-            (((int64_t)nb * g.d + z0 + iz) * g.h + y0 + iy) * g.w + x0 + 16 * xt + 4 * kg;
+        bf16_t e[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t mo = mb + i;
-            float v = acc[t][i] + bv;
-            if (add)
-                v = v + bf2f(add[mo * 16 + r]);
-            if (relu)
-                v = v > 0.f ? v : 0.f;
-            out[mo * 16 + r] = f2bf(v);
+        for (int j = 0; j < 8; ++j) {
+            const int tp = 8 * kg + j;
+            const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+            const int idx = ((iz + dz + 1) * SY + iy + dy + 1) * SX + 16 * xt + r + dx + 1;
+            e[j] = tp < 27 ? sm[tp < 27 ? idx : 0] : (bf16_t)0;
         }
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, qb), pack8(e), f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
+    conv_tile_store<NT, XT, LPW>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, nullptr, relu,
+                                 out);
 }
 
 // ---- weight gradient: per (voxel chunk, tap) workgroup, LDS-staged MFMA ----
@@ -528,6 +627,41 @@ __global__ void k_head_out(int64_t V, int nv, int C, const bf16_t *__restrict__ 
     }
 }
 
+// the same outputs for nv | 256 and C % 8 == 0: a block writes 256 consecutive outputs
+// (256 / nv voxels, all nv RHS); each thread computes one (voxel, RHS) from its C channels
+// read as 16-byte vectors (the block's reads of one sample are contiguous) and the block
+// writes its outputs in order through LDS.  Per-output arithmetic as k_head_out.
+__global__ __launch_bounds__(256) void k_head_out_t(int64_t V, int nv, int C,
+                                                    const bf16_t *__restrict__ h,
+                                                    const float *__restrict__ w,
+                                                    const float *__restrict__ b,
+                                                    const float *__restrict__ scale, float o,
+                                                    double *__restrict__ x0) {
+    __shared__ double sm[256];
+    const int t = threadIdx.x, vb = 256 / nv;
+    const int64_t i0 = (int64_t)blockIdx.x * 256, v0 = i0 / nv;
+    const int k = t / vb, dv = t - k * vb;
+    const int64_t v = v0 + dv;
+    if (v < V) {
+        const uint4 *hp = reinterpret_cast<const uint4 *>(h + ((int64_t)k * V + v) * C);
+        float s = 0.f;
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const uint4 q = hp[c8];
+            const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s = s + w[8 * c8 + 2 * j] * bf2f((bf16_t)(u[j] & 0xffffu));
+                s = s + w[8 * c8 + 2 * j + 1] * bf2f((bf16_t)(u[j] >> 16));
+            }
+        }
+        const float y = s + b[0];
+        sm[dv * nv + k] = (double)scale[k] * (double)o * (double)y;
+    }
+    __syncthreads();
+    if (i0 + t < V * nv)
+        x0[i0 + t] = sm[t];
+}
+
 // head gradients: gh[m][c] = gy[m] w[c]; per-chunk partials of gw[c] = sum gy h, gb = sum gy
 __global__ void k_head_bwd(int64_t M, int C, const bf16_t *__restrict__ h, const float *__restrict__ w,
                            const float *__restrict__ gy, bf16_t *__restrict__ gh, int64_t chunk,
@@ -611,41 +745,83 @@ __global__ void k_adam(int64_t n, float *__restrict__ p, const float *__restrict
 }
 
 // per-sample RMS of an fp64 [vox][nv] multivector slice: scale[k] (partials per chunk)
-__global__ void k_rms_part(int64_t V, int nv, const double *__restrict__ b, int64_t chunk,
-                           double *__restrict__ part) {
+// per block: rows [c0, c1) of the interleaved [V][nv] multivector; thread t sums column
+// k = t % nv over the rows c0 + q, c0 + q + RL, ... (q = t / nv, RL = 256 / nv phases) and
+// the block partial adds the phase sums in the order q = 0, 1, ... (nv <= 64)
+__global__ __launch_bounds__(256) void k_rms_part(int64_t V, int nv, const double *__restrict__ b,
+                                                  int64_t chunk, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int t = threadIdx.x, RL = 256 / nv, k = t % nv, q = t / nv;
     const int64_t c0 = (int64_t)blockIdx.x * chunk, c1 = std::min<int64_t>(V, c0 + chunk);
-    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
-        double s = 0.0;
-        for (int64_t i = c0; i < c1; ++i)
-            s = s + b[i * nv + k] * b[i * nv + k];
-        part[(size_t)blockIdx.x * nv + k] = s;
+    double s = 0.0;
+    if (q < RL) {
+#pragma unroll 8
+        for (int64_t i = c0 + q; i < c1; i += RL) {
+            const double x = b[i * nv + k];
+            s = s + x * x;
+        }
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < nv) {
+        double a = 0.0;
+        for (int j = 0; j < RL; ++j)
+            a = a + red[j * nv + t];
+        part[(size_t)blockIdx.x * nv + t] = a;
     }
 }
 
-__global__ void k_rms_fin(int nchunk, int nv, int64_t V, const double *__restrict__ part,
-                          float *__restrict__ scale) {
-    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; ++c)
+// scale[k] = rms of column k: the chunk partials summed in the same phase order
+__global__ __launch_bounds__(256) void k_rms_fin(int nchunk, int nv, int64_t V,
+                                                 const double *__restrict__ part,
+                                                 float *__restrict__ scale) {
+    __shared__ double red[256];
+    const int t = threadIdx.x, RL = 256 / nv, k = t % nv, q = t / nv;
+    double s = 0.0;
+    if (q < RL)
+        for (int c = q; c < nchunk; c += RL)
             s = s + part[(size_t)c * nv + k];
-        const double r = sqrt(s / (double)V);
-        scale[k] = r > 0.0 ? (float)r : 1.f;
+    red[t] = s;
+    __syncthreads();
+    if (t < nv) {
+        double a = 0.0;
+        for (int j = 0; j < RL; ++j)
+            a = a + red[j * nv + t];
+        const double r = sqrt(a / (double)V);
+        scale[t] = r > 0.0 ? (float)r : 1.f;
     }
 }
 
 // network input (sample k = batch index) from column k of an interleaved [vox][nv] fp64
-// multivector: in[k][vox] = b[vox][k] / scale[k] (bf16, 1 channel); tgt likewise (fp32)
-__global__ void k_gather_in(int64_t V, int nv, const double *__restrict__ b,
-                            const float *__restrict__ scale, bf16_t *__restrict__ in,
-                            const double *__restrict__ x, float *__restrict__ tgt) {
-    const int64_t n = V * nv;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / V);
-        const int64_t v = i % V;
-        in[i] = f2bf((float)(b[v * nv + k] / (double)scale[k]));
+// multivector: in[k][vox] = b[vox][k] / scale[k] (bf16, 1 channel); tgt likewise (fp32).
+// A block transposes 64 voxels x nv (nv <= 64) through LDS: the multivector rows are read
+// contiguously and each sample's 64 voxels are written contiguously.
+__global__ __launch_bounds__(256) void k_gather_in(int64_t V, int nv, const double *__restrict__ b,
+                                                   const float *__restrict__ scale,
+                                                   bf16_t *__restrict__ in,
+                                                   const double *__restrict__ x,
+                                                   float *__restrict__ tgt) {
+    constexpr int P = 65; // padded LDS row
+    __shared__ bf16_t sb[64 * P];
+    __shared__ float sx[64 * P];
+    const int t = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int nvox = (int)std::min<int64_t>(64, V - v0);
+    for (int e = t; e < nvox * nv; e += 256) { // e = dv * nv + k, contiguous in b
+        const int dv = e / nv, k = e - dv * nv;
+        const size_t gi = (size_t)v0 * nv + e;
+        sb[k * P + dv] = f2bf((float)(b[gi] / (double)scale[k]));
         if (tgt)
-            tgt[i] = (float)(x[v * nv + k] / (double)scale[k]);
+            sx[k * P + dv] = (float)(x[gi] / (double)scale[k]);
+    }
+    __syncthreads();
+    for (int e = t; e < nv * 64; e += 256) { // e = k * 64 + dv, contiguous in `in`
+        const int k = e >> 6, dv = e & 63;
+        if (dv < nvox) {
+            in[(size_t)k * V + v0 + dv] = sb[k * P + dv];
+            if (tgt)
+                tgt[(size_t)k * V + v0 + dv] = sx[k * P + dv];
+        }
     }
 }
 
@@ -926,10 +1102,17 @@ void launch_conv_tile(const Dims &g, const bf16_t *in, const bf16_t *wt, int kpa
 // up: `in` is the coarse input of a nearest x2 upsampling onto g (fused)
 void conv_fwd(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *in, const bf16_t *add,
               bool relu, bf16_t *out, bool up = false) {
-    // 16 output channels on volumes the tile divides: LDS-staged tiles (16 input channels,
-    // or 16 / 32 through the fused upsampling, whose coarse tile is small)
-    if (g_conv_tile && c.cout == 16 && g.w % kTX == 0 && g.h % kTY == 0 && g.d % kTZ == 0 &&
-        (c.cin == 16 || (up && c.cin == 32))) {
+    // 16 output channels on volumes the tile divides: LDS-staged tiles (1 or 16 input
+    // channels, or 16 / 32 through the fused upsampling, whose coarse tile is small)
+    const bool tiles = g_conv_tile && c.cout == 16 && g.w % kTX == 0 && g.h % kTY == 0 &&
+                       g.d % kTZ == 0;
+    if (tiles && c.cin == 1 && !up && !add) {
+        const dim3 grid((unsigned)(g.vox() / (kTX * kTY * kTZ))), blk(256);
+        hipLaunchKernelGGL(k_conv_tile1, grid, blk, 0, U->stream, g, in, c.wt, c.kpad, c.b,
+                           (int)relu, out);
+        return;
+    }
+    if (tiles && (c.cin == 16 || (up && c.cin == 32))) {
         if (c.cin == 16 && !up)
             launch_conv_tile<2, false>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
         else if (c.cin == 16)
@@ -1311,8 +1494,8 @@ extern "C" int xh_unet_loss_grad(xh_unet U, int n, int d, int h, int w, const fl
 static void column_rms(xh_unet_s *U, int64_t V, int nv, const double *b) {
     const int64_t cb = (V + 1023) / 1024;
     const int nb = (int)((V + cb - 1) / cb);
-    hipLaunchKernelGGL(k_rms_part, dim3(nb), dim3(64), 0, U->stream, V, nv, b, cb, U->dpart);
-    hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(64), 0, U->stream, nb, nv, V, U->dpart, U->scale);
+    hipLaunchKernelGGL(k_rms_part, dim3(nb), dim3(256), 0, U->stream, V, nv, b, cb, U->dpart);
+    hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(256), 0, U->stream, nb, nv, V, U->dpart, U->scale);
 }
 
 extern "C" int xh_unet_train(xh_unet U, int nx, int ny, int nz, int nsets,
@@ -1342,7 +1525,7 @@ extern "C" int xh_unet_train(xh_unet U, int nx, int ny, int nz, int nsets,
     };
     for (int q = 0; q < nsets; ++q) {
         column_rms(U, V, nv, db[q]);
-        hipLaunchKernelGGL(k_gather_in, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv,
+        hipLaunchKernelGGL(k_gather_in, dim3((unsigned)((V + 63) / 64)), dim3(256), 0, U->stream, V, nv,
                            db[q], U->scale, in_all + (size_t)q * nv * V, dx[q],
                            tgt_all + (size_t)q * nv * V);
     }
@@ -1413,13 +1596,17 @@ extern "C" int xh_unet_infer(xh_unet U, int nx, int ny, int nz, int nv, const do
     if (rc)
         return rc;
     column_rms(U, V, nv, db);
-    hipLaunchKernelGGL(k_gather_in, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, db,
+    hipLaunchKernelGGL(k_gather_in, dim3((unsigned)((V + 63) / 64)), dim3(256), 0, U->stream, V, nv, db,
                        U->scale, W->in[0], nullptr, nullptr);
     forward(U, *W, true);
     // x0 = s * o * (head of the top decoder level)
     const bf16_t *h0 = U->L == 1 ? W->s[0] : W->hh[0];
-    hipLaunchKernelGGL(k_head_out, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv, U->hc,
-                       h0, U->hw, U->hb, U->scale, U->oscale, dx0);
+    if (256 % nv == 0 && U->hc % 8 == 0)
+        hipLaunchKernelGGL(k_head_out_t, dim3((unsigned)((V * nv + 255) / 256)), dim3(256), 0,
+                           U->stream, V, nv, U->hc, h0, U->hw, U->hb, U->scale, U->oscale, dx0);
+    else
+        hipLaunchKernelGGL(k_head_out, dim3(grid1(V * nv)), dim3(256), 0, U->stream, V, nv,
+                           U->hc, h0, U->hw, U->hb, U->scale, U->oscale, dx0);
     UN_HIP(hipGetLastError());
     UN_HIP(hipStreamSynchronize(U->stream));
     return 0;

@@ -156,3 +156,31 @@ def test_tiled_convolution_is_identical(gpu):
     finally:
         X.set_option("unet_conv_tile", 1)
         X.set_option("unet_fuse_up", 1)
+
+
+@pytest.mark.parametrize("nv", [2, 3])
+def test_inference_matches_oracle_forward(gpu, nv):
+    # x0[:, k] = rms(b[:, k]) * o * net(b[:, k] / rms(b[:, k])) on a grid the LDS-staged
+    # convolution tiles cover (the single-channel input layer included); nv = 2 takes the
+    # vector head, nv = 3 the per-element one
+    X = gpu
+    channels = (16, 32)
+    net = X.UNet(channels, seed=9)
+    p = net.get_params()
+    rng = np.random.default_rng(4)
+    for sl in layer_slices(channels)[1::2]:
+        p[sl] = rng.standard_normal(sl.stop - sl.start).astype(np.float32) * 0.1
+    net.set_params(p)
+    net.output_scale = 0.7
+    nx, ny, nz = 64, 16, 8
+    V = nx * ny * nz
+    b = rng.standard_normal((V, nv))
+    x0 = X.DeviceArray(V * nv)
+    net.infer(nx, ny, nz, nv, X.DeviceArray.from_host(b.ravel()), x0)
+    got = x0.to_host().reshape(V, nv)
+    scale = np.sqrt((b ** 2).mean(0))
+    inp = (b / scale).T.reshape(nv, nz, ny, nx)
+    ref = UO.UNet(channels, p.astype(np.float64))
+    _, _, ry = ref.loss_grad(inp, np.zeros_like(inp), use_bf16=True)
+    exp = (scale[:, None] * 0.7 * ry.reshape(nv, V)).T
+    assert np.linalg.norm(got - exp) <= 1e-2 * np.linalg.norm(exp)
