@@ -195,61 +195,70 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
         }
 }
 
-// ---- forward convolution over LDS-staged tiles (cout = 16) ----
-// A workgroup computes kTZ x kTY lines of kTX voxels.  The input those voxels read (the
-// tile with a one-voxel halo, or with UP the coarse voxels under it) is staged in LDS once,
-// zero outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes)
-// by voxel, so the 16 rows of an A fragment are 256 contiguous bytes.  All 27 taps then
-// read their input fragments from LDS instead of gathering them through the L1 per tap; the
-// weight fragments of all k steps are loaded while the tile is staged.  The weights are the
-// A operand (the MFMA gives the transposed output tile, 4 channels of a voxel per lane, so
-// the stores are 8-byte vectors); the products, their k order and the epilogue's arithmetic
-// are those of k_conv, so the output is bit-identical.  The launch requires w % kTX == 0,
-// h % kTY == 0 and d % kTZ == 0.
+// ---- forward convolution over LDS-staged tiles (cout = 16 * CT) ----
+// A workgroup computes kTZ x TY lines of kTX voxels.  The input those voxels read (the tile
+// with a one-voxel halo, or with UP the coarse voxels under it) is staged in LDS once, zero
+// outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes) by
+// voxel, so the 16 rows of an input fragment are 256 contiguous bytes.  All 27 taps then
+// read their input fragments from LDS instead of gathering them through the L1 per tap.
+// With CT = 1 the weight fragments of all k steps are loaded while the tile is staged; with
+// CT > 1 (CT fragments per step) each step's are loaded one step ahead.  The weights are
+// the A operand (the MFMA gives the transposed output tile, 4 channels of a voxel per lane,
+// so the stores are 8-byte vectors); the products, their k order and the epilogue's
+// arithmetic are those of k_conv, so the output is bit-identical.  The launch requires
+// w % kTX == 0, h % TY == 0 and d % kTZ == 0.
 constexpr int kTX = 64, kTY = 4, kTZ = 2;
 
-// Epilogue of the tiled convolutions (weights as the A operand): lane (r, kg) of tile t
-// holds channels 4kg..4kg+3 of voxel r, written (and `add` read) as 8 bytes.  Per element
-// act(acc + bias + add), as in k_conv.
-template <int NT, int XT, int LPW>
+// Epilogue of the tiled convolutions: lane (r, kg) of tile t, channel tile c holds
+// channels 16c + 4kg .. 16c + 4kg + 3 of voxel r, written (and `add` read) as 8 bytes.
+// Per element act(acc + bias + add), as in k_conv.
+template <int CT, int NT, int XT, int LPW, int TY>
 __device__ __forceinline__ void conv_tile_store(const Dims &g, int nb, int z0, int y0, int x0,
-                                                int wave, int r, int kg, const f32x4 (&acc)[NT],
+                                                int wave, int r, int kg,
+                                                const f32x4 (&acc)[NT][CT],
                                                 const float *__restrict__ bias,
                                                 const bf16_t *__restrict__ add, int relu,
                                                 bf16_t *__restrict__ out) {
-    float bv[4];
+    constexpr int COUT = 16 * CT;
+    float bv[CT][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        bv[i] = bias ? bias[4 * kg + i] : 0.f;
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            bv[c][i] = bias ? bias[16 * c + 4 * kg + i] : 0.f;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int li = wave * LPW + t / XT, xt = t % XT;
-        const int iz = li / kTY, iy = li % kTY;
+        const int iz = li / TY, iy = li % TY;
         const int64_t mo =
             (((int64_t)nb * g.d + z0 + iz) * g.h + y0 + iy) * g.w + x0 + 16 * xt + r;
-        float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            v[i] = acc[t][i] + bv[i];
-        if (add) {
-            const uint2 a = *reinterpret_cast<const uint2 *>(add + mo * 16 + 4 * kg);
-            v[0] = v[0] + bf2f((bf16_t)(a.x & 0xffffu));
-            v[1] = v[1] + bf2f((bf16_t)(a.x >> 16));
-            v[2] = v[2] + bf2f((bf16_t)(a.y & 0xffffu));
-            v[3] = v[3] + bf2f((bf16_t)(a.y >> 16));
-        }
-        if (relu)
+        for (int c = 0; c < CT; ++c) {
+            const int64_t o0 = mo * COUT + 16 * c + 4 * kg;
+            float v[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                v[i] = v[i] > 0.f ? v[i] : 0.f;
-        uint2 o;
-        o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-        o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-        *reinterpret_cast<uint2 *>(out + mo * 16 + 4 * kg) = o;
+                v[i] = acc[t][c][i] + bv[c][i];
+            if (add) {
+                const uint2 a = *reinterpret_cast<const uint2 *>(add + o0);
+                v[0] = v[0] + bf2f((bf16_t)(a.x & 0xffffu));
+                v[1] = v[1] + bf2f((bf16_t)(a.x >> 16));
+                v[2] = v[2] + bf2f((bf16_t)(a.y & 0xffffu));
+                v[3] = v[3] + bf2f((bf16_t)(a.y >> 16));
+            }
+            if (relu)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    v[i] = v[i] > 0.f ? v[i] : 0.f;
+            uint2 o;
+            o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+            o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+            *reinterpret_cast<uint2 *>(out + o0) = o;
+        }
     }
 }
 
-template <int NCG, bool UP>
+template <int NCG, int CT, int TY, bool UP>
 __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restrict__ in,
                                                    const bf16_t *__restrict__ wt, int kpad,
                                                    const float *__restrict__ bias,
@@ -257,16 +266,19 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
                                                    bf16_t *__restrict__ out) {
     constexpr int CIN = 8 * NCG;
     constexpr int SZ = UP ? kTZ / 2 + 2 : kTZ + 2;
-    constexpr int SY = UP ? kTY / 2 + 2 : kTY + 2;
+    constexpr int SY = UP ? TY / 2 + 2 : TY + 2;
     constexpr int SX = UP ? kTX / 2 + 2 : kTX + 2;
     constexpr int NCH = SZ * SY * SX * NCG; // 16-byte chunks staged
     constexpr int PER = (NCH + 255) / 256;
     constexpr int XT = kTX / 16;              // 16-voxel tiles per line
-    constexpr int LPW = kTZ * kTY / 4;        // lines per wave
+    constexpr int LPW = kTZ * TY / 4;         // lines per wave
     constexpr int NT = LPW * XT;              // tiles per wave
+    constexpr int KS = (27 * CIN + 31) / 32; // k steps (kpad == 32 * KS)
+    constexpr bool PRE = CT == 1;            // all weight fragments preloaded
+    static_assert(LPW >= 1 && (kTZ * TY) % 4 == 0, "tile lines must split over 4 waves");
     __shared__ uint4 sm[NCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ntx = g.w / kTX, nty = g.h / kTY, ntz = g.d / kTZ;
+    const int ntx = g.w / kTX, nty = g.h / TY, ntz = g.d / kTZ;
     int64_t bid = blockIdx.x;
     const int tx = (int)(bid % ntx);
     bid /= ntx;
@@ -274,15 +286,17 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     bid /= nty;
     const int tz = (int)(bid % ntz);
     const int nb = (int)(bid / ntz);
-    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
-    // the staged volume (coarse with UP) and the tile's origin in it
-    // every k step's weight fragment, loaded while the tile is staged
-    constexpr int KS = (27 * CIN + 31) / 32; // kpad == 32 * KS
+    const int x0 = tx * kTX, y0 = ty * TY, z0 = tz * kTZ;
     const int r = lane & 15, kg = lane >> 4;
-    uint4 qb[KS];
+    // lane's weight row (output channel 16c + r of channel tile c at + 16c * kpad)
+    const bf16_t *wrow = wt + (size_t)r * kpad + 8 * kg;
+    uint4 qa[PRE ? KS : 1];
+    if constexpr (PRE) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-        qb[ks] = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + 32 * ks + 8 * kg);
+        for (int ks = 0; ks < KS; ++ks)
+            qa[ks] = *reinterpret_cast<const uint4 *>(wrow + 32 * ks);
+    }
+    // the staged volume (coarse with UP) and the tile's origin in it
     const int sd = UP ? g.d / 2 : g.d, sh = UP ? g.h / 2 : g.h, sw = UP ? g.w / 2 : g.w;
     const int oz = (UP ? z0 / 2 : z0) - 1, oy = (UP ? y0 / 2 : y0) - 1,
               ox = (UP ? x0 / 2 : x0) - 1;
@@ -312,23 +326,30 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
             }
         }
     }
+    uint4 qn[CT];
+    if constexpr (!PRE) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+            qn[c] = *reinterpret_cast<const uint4 *>(wrow + (size_t)16 * c * kpad);
+    }
     __syncthreads();
-    f32x4 acc[NT];
+    f32x4 acc[NT][CT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-        acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+            acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
     constexpr int K = 27 * CIN;
     // tap and channel of the lane's first k, advanced by 32 per step (as in k_conv)
     int tt = (8 * kg) / CIN, cc = 8 * kg - tt * CIN;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
+    auto step = [&](int ks, const uint4 (&qb)[CT]) {
         const bool kok = 32 * ks + 8 * kg < K;
         const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
         const int cg = cc >> 3;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int li = wave * LPW + t / XT, xt = t % XT;
-            const int iz = li / kTY, iy = li % kTY;
+            const int iz = li / TY, iy = li % TY;
             int sz, sy, sx;
             if constexpr (UP) {
                 // fine offset -1 maps to coarse -1 (staged zero), as the fine bounds test
@@ -344,16 +365,40 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
             if (!kok)
                 q = make_uint4(0, 0, 0, 0);
             // weights as the A operand: D[co][voxel], each lane 4 channels of one voxel
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                __builtin_bit_cast(bf16x8, qb[ks]), __builtin_bit_cast(bf16x8, q), acc[t], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8, qb[c]), __builtin_bit_cast(bf16x8, q), acc[t][c],
+                    0, 0, 0);
         }
         cc += 32;
         while (cc >= CIN) {
             cc -= CIN;
             ++tt;
         }
+    };
+    if constexpr (PRE) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const uint4 qb[1] = {qa[ks]};
+            step(ks, qb);
+        }
+    } else {
+        for (int ks = 0; ks < KS; ++ks) {
+            uint4 qb[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                qb[c] = qn[c];
+            if (ks + 1 < KS)
+#pragma unroll
+                for (int c = 0; c < CT; ++c)
+                    qn[c] = *reinterpret_cast<const uint4 *>(wrow + (size_t)16 * c * kpad +
+                                                             32 * (ks + 1));
+            step(ks, qb);
+        }
     }
-    conv_tile_store<NT, XT, LPW>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, add, relu, out);
+    conv_tile_store<CT, NT, XT, LPW, TY>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, add, relu,
+                                         out);
 }
 
 // The single-channel input layer (cin = 1, K = 27 taps in one k step of 32) over the same
@@ -402,7 +447,7 @@ __global__ __launch_bounds__(256) void k_conv_tile1(Dims g, const bf16_t *__rest
     __syncthreads();
     const int r = lane & 15, kg = lane >> 4;
     const uint4 qb = *reinterpret_cast<const uint4 *>(wt + (size_t)r * kpad + 8 * kg);
-    f32x4 acc[NT];
+    f32x4 acc[NT][1];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int li = wave * LPW + t / XT, xt = t % XT;
@@ -415,11 +460,11 @@ __global__ __launch_bounds__(256) void k_conv_tile1(Dims g, const bf16_t *__rest
             const int idx = ((iz + dz + 1) * SY + iy + dy + 1) * SX + 16 * xt + r + dx + 1;
             e[j] = tp < 27 ? sm[tp < 27 ? idx : 0] : (bf16_t)0;
         }
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+        acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
             __builtin_bit_cast(bf16x8, qb), pack8(e), f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
-    conv_tile_store<NT, XT, LPW>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, nullptr, relu,
-                                 out);
+    conv_tile_store<1, NT, XT, LPW, kTY>(g, nb, z0, y0, x0, wave, r, kg, acc, bias, nullptr,
+                                         relu, out);
 }
 
 // ---- weight gradient: per (voxel chunk, tap) workgroup, LDS-staged MFMA ----
@@ -1090,36 +1135,59 @@ void launch_conv(const Dims &g, const bf16_t *in, int cin, const bf16_t *wt, int
 // 0: every forward convolution runs as k_conv (no LDS-staged tiles)
 int g_conv_tile = 1;
 
-template <int NCG, bool UP>
+template <int NCG, int CT, int TY, bool UP>
 void launch_conv_tile(const Dims &g, const bf16_t *in, const bf16_t *wt, int kpad,
                       const float *bias, const bf16_t *add, int relu, bf16_t *out,
                       hipStream_t s) {
-    const dim3 grid((unsigned)(g.vox() / (kTX * kTY * kTZ))), blk(256);
-    hipLaunchKernelGGL((k_conv_tile<NCG, UP>), grid, blk, 0, s, g, in, wt, kpad, bias, add,
-                       relu, out);
+    const dim3 grid((unsigned)(g.vox() / (kTX * TY * kTZ))), blk(256);
+    hipLaunchKernelGGL((k_conv_tile<NCG, CT, TY, UP>), grid, blk, 0, s, g, in, wt, kpad, bias,
+                       add, relu, out);
 }
 
 // up: `in` is the coarse input of a nearest x2 upsampling onto g (fused)
 void conv_fwd(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *in, const bf16_t *add,
               bool relu, bf16_t *out, bool up = false) {
-    // 16 output channels on volumes the tile divides: LDS-staged tiles (1 or 16 input
-    // channels, or 16 / 32 through the fused upsampling, whose coarse tile is small)
-    const bool tiles = g_conv_tile && c.cout == 16 && g.w % kTX == 0 && g.h % kTY == 0 &&
-                       g.d % kTZ == 0;
-    if (tiles && c.cin == 1 && !up && !add) {
+    // 16 or 32 output channels on volumes the tiles divide: LDS-staged tiles (the staged
+    // input at most 68 KB: 32 plain input channels take tiles of 2 lines)
+    const bool tiles = g_conv_tile && g.w % kTX == 0 && g.h % kTY == 0 && g.d % kTZ == 0;
+    hipStream_t st = U->stream;
+    const int rl = (int)relu;
+    if (tiles && c.cout == 16 && c.cin == 1 && !up && !add) {
         const dim3 grid((unsigned)(g.vox() / (kTX * kTY * kTZ))), blk(256);
-        hipLaunchKernelGGL(k_conv_tile1, grid, blk, 0, U->stream, g, in, c.wt, c.kpad, c.b,
-                           (int)relu, out);
+        hipLaunchKernelGGL(k_conv_tile1, grid, blk, 0, st, g, in, c.wt, c.kpad, c.b, rl, out);
         return;
     }
-    if (tiles && (c.cin == 16 || (up && c.cin == 32))) {
-        if (c.cin == 16 && !up)
-            launch_conv_tile<2, false>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
-        else if (c.cin == 16)
-            launch_conv_tile<2, true>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
-        else
-            launch_conv_tile<4, true>(g, in, c.wt, c.kpad, c.b, add, (int)relu, out, U->stream);
-        return;
+    if (tiles && c.cout == 16) {
+        if (c.cin == 16 && !up) {
+            launch_conv_tile<2, 1, 4, false>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 16 && up) {
+            launch_conv_tile<2, 1, 4, true>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 32 && up) {
+            launch_conv_tile<4, 1, 4, true>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+    }
+    if (tiles && c.cout == 32) {
+        if (c.cin == 16 && !up) {
+            launch_conv_tile<2, 2, 4, false>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 32 && !up) {
+            launch_conv_tile<4, 2, 2, false>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 32 && up) {
+            launch_conv_tile<4, 2, 4, true>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 48 && up) {
+            launch_conv_tile<6, 2, 4, true>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
     }
 #define XH_CONV_CASE(CTV)                                                                      \
     if (up)                                                                                    \
