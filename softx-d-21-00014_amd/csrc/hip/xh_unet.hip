@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
 // outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes) by
 // voxel, so the 16 rows of an input fragment are 256 contiguous bytes.  All 27 taps then
 // read their input fragments from LDS instead of gathering them through the L1 per tap.
-// With CT = 1 the weight fragments of all k steps are loaded while the tile is staged; with
-// CT > 1 (CT fragments per step) each step's are loaded one step ahead.  The weights are
+// With CT = 1 and at most 16 k steps the weight fragments of all steps are loaded after
+// the staging stores; otherwise each step's CT fragments are loaded one step ahead.  The weights are
 // the A operand (the MFMA gives the transposed output tile, 4 channels of a voxel per lane,
 // so the stores are 8-byte vectors); the products, their k order and the epilogue's
 // arithmetic are those of k_conv, so the output is bit-identical.  The launch requires
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     constexpr int LPW = kTZ * TY / 4;         // lines per wave
     constexpr int NT = LPW * XT;              // tiles per wave
     constexpr int KS = (27 * CIN + 31) / 32; // k steps (kpad == 32 * KS)
-    constexpr bool PRE = CT == 1;            // all weight fragments preloaded
+    constexpr bool PRE = CT == 1 && KS <= 16; // all weight fragments preloaded
     static_assert(LPW >= 1 && (kTZ * TY) % 4 == 0, "tile lines must split over 4 waves");
     __shared__ uint4 sm[NCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -290,12 +290,6 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     const int r = lane & 15, kg = lane >> 4;
     // lane's weight row (output channel 16c + r of channel tile c at + 16c * kpad)
     const bf16_t *wrow = wt + (size_t)r * kpad + 8 * kg;
-    uint4 qa[PRE ? KS : 1];
-    if constexpr (PRE) {
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-            qa[ks] = *reinterpret_cast<const uint4 *>(wrow + 32 * ks);
-    }
     // the staged volume (coarse with UP) and the tile's origin in it
     const int sd = UP ? g.d / 2 : g.d, sh = UP ? g.h / 2 : g.h, sw = UP ? g.w / 2 : g.w;
     const int oz = (UP ? z0 / 2 : z0) - 1, oy = (UP ? y0 / 2 : y0) - 1,
@@ -325,6 +319,13 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
                 sm[(line * NCG + cg) * SX + sx] = st[j];
             }
         }
+    }
+    // weight fragments, issued once the staged values have left the registers
+    uint4 qa[PRE ? KS : 1];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+            qa[ks] = *reinterpret_cast<const uint4 *>(wrow + 32 * ks);
     }
     uint4 qn[CT];
     if constexpr (!PRE) {
