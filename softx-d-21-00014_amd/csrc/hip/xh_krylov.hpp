@@ -51,6 +51,8 @@ __device__ inline void finalize_body(int op, SolverState *__restrict__ st,
     if (tid < C * S) {
         const int c = tid % C, sub = tid / C;
         double acc = 0.0;
+        // unrolled so that many partial loads are in flight; the additions keep their order
+#pragma unroll 16
         for (int g = sub; g < nparts; g += S)
             acc = acc + partials[(size_t)g * stride + c];
         red[sub * C + c] = acc;
