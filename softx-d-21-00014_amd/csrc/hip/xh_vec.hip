@@ -372,6 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(int64_t n, int nv, const VT *_
     const int64_t i = e / nv;
     const int k = (int)(e % nv);
     VT acc = VT(0);
+    // j order as dense_Ax_y; unrolled so that many loads are in flight
+#pragma unroll 16
     for (int64_t j = 0; j < n; ++j)
         acc = acc + D[i * n + j] * x[j * nv + k];
     y[e] = acc;
