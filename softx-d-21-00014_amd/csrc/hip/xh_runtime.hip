@@ -312,6 +312,61 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
     return 0;
 }
 
+// The core-block part of h for hybrid GS with blocks of B rows: row i keeps the entries
+// whose column lies in [B*(i/B), B*(i/B) + B), in stored order (the only ones the sweep of
+// csr_SGS on the core block reads, csr_gs.inl:75-130), with u32 columns and the values
+// in the matrix precision; inv_diag is taken from A (not owned).
+int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
+    M = Mat();
+    const int64_t n = h.nrows;
+    std::vector<uint32_t> row(n + 1, 0), col;
+    std::vector<double> val;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t lo = (i / B) * B, hi = std::min(n, lo + B);
+        for (uint32_t j = h.row[i]; j < h.row[i + 1]; ++j)
+            if ((int64_t)h.col[j] >= lo && (int64_t)h.col[j] < hi) {
+                col.push_back(h.col[j]);
+                val.push_back(h.val[j]);
+            }
+        row[i + 1] = (uint32_t)col.size();
+    }
+    const int64_t nnz = (int64_t)col.size();
+    M.nrows = M.ncols = n;
+    M.nnz = nnz;
+    M.prec = prec;
+    M.idx = IDX_U32;
+    size_t acc = 0;
+    int rc;
+    uint32_t *dc = nullptr;
+    if ((rc = dmalloc(&M.row, n + 1, acc)) || (rc = dmalloc(&dc, std::max<int64_t>(nnz, 1), acc)))
+        return rc;
+    M.col = dc;
+    XH_HIP(hipMemcpy(M.row, row.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+    if (nnz)
+        XH_HIP(hipMemcpy(dc, col.data(), nnz * 4, hipMemcpyHostToDevice));
+    if (prec == F64) {
+        double *v = nullptr;
+        if ((rc = dmalloc(&v, std::max<int64_t>(nnz, 1), acc)))
+            return rc;
+        M.val = v;
+        if (nnz)
+            XH_HIP(hipMemcpy(v, val.data(), nnz * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> vf(nnz);
+        for (int64_t j = 0; j < nnz; ++j)
+            vf[j] = (float)val[j]; // as upload_mat converts A
+        float *v = nullptr;
+        if ((rc = dmalloc(&v, std::max<int64_t>(nnz, 1), acc)))
+            return rc;
+        M.val = v;
+        if (nnz)
+            XH_HIP(hipMemcpy(v, vf.data(), nnz * 4, hipMemcpyHostToDevice));
+    }
+    M.inv_diag = A.inv_diag;
+    M.bytes = acc;
+    return 0;
+}
+
 void free_mat(Mat &M) {
     for (void *p : {(void *)M.row, M.col, (void *)M.panel_base, M.val, M.inv_diag,
                     M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind, (void *)M.gcode})
@@ -439,6 +494,10 @@ struct Level {
     bool own_A = true;
     double mn = 0, mx = 0;
     int64_t hsgs_B = 0;
+    // the entries of A inside each row's core block of hsgs_B rows (stored order, u32
+    // columns; inv_diag is A's): what the hybrid GS sweeps read, built when a role of
+    // the level is HSGS
+    Mat core;
     void *tmp = nullptr, *tmp_t = nullptr, *b = nullptr, *x = nullptr;
     void *sr = nullptr, *su1 = nullptr, *su2 = nullptr;
     void *dinv = nullptr;
@@ -551,14 +610,15 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
     case XH_HSGS: {
         const double relax = (double)role.relax_factor;
         for (int it = 0; it < std::max(1, role.max_iters); ++it) {
+            const Mat &G = L.core.row ? L.core : L.A;
             if (x_zero) {
-                hsgs_sweeps(L.A, nv, b, x, 1, relax, L.hsgs_B, S->dflag, st);
+                hsgs_sweeps(G, nv, b, x, 1, relax, L.hsgs_B, S->dflag, st);
             } else {
                 SpmvArgs a = args(S);
                 a.b = b;
                 a.blk_rows = L.hsgs_B;
                 spmv(L.A, nv, x, L.sr, OP_OFFBLOCK, a, st);
-                hsgs_sweeps(L.A, nv, L.sr, x, 0, relax, L.hsgs_B, S->dflag, st);
+                hsgs_sweeps(G, nv, L.sr, x, 0, relax, L.hsgs_B, S->dflag, st);
             }
             x_zero = false;
         }
@@ -1323,6 +1383,12 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
                 cheby_coeffs(L.mn, L.mx, roles[q]->poly_order, (double)roles[q]->spectrum_fraction,
                              *dst[q]))
                 return bail(fail(XH_EINVAL, "Chebyshev order out of range"));
+        if (roles[0]->method == XH_HSGS || roles[1]->method == XH_HSGS ||
+            roles[2]->method == XH_HSGS) {
+            if ((rc = upload_core(H->A[l], L.hsgs_B, L.prec, L.A, L.core)))
+                return bail(rc);
+            S->bytes += L.core.bytes;
+        }
     }
     // coarsest direct solve: dense Gauss-Jordan inverse on the host (Direct::init)
     {
@@ -1480,6 +1546,8 @@ int xh_solver_destroy(xh_solver S) {
     for (Level &L : S->lv) {
         if (L.own_A)
             free_mat(L.A);
+        L.core.inv_diag = nullptr; // A's
+        free_mat(L.core);
         free_mat(L.P);
         free_mat(L.R);
         for (void *q : {L.tmp, L.tmp_t, L.b, L.x, L.sr, L.su1, L.su2, L.dinv, L.cb64, L.cx64})

@@ -306,6 +306,9 @@ __global__ __launch_bounds__(64) void k_hsgs_lds(const uint32_t *__restrict__ ro
             if constexpr (sizeof(IT) == 2)
                 base = pbase[i >> shift];
             VT sum = t[(size_t)i * NV + k];
+            // unrolled so that the row's column and value loads are in flight together;
+            // the subtractions keep the stored order
+#pragma unroll 8
             for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
                 const int64_t c = (int64_t)(base + (uint32_t)col[j]);
                 if (c >= lo && c < hi)
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(64) void k_hsgs_glb(const uint32_t *__restrict__ ro
             if constexpr (sizeof(IT) == 2)
                 base = pbase[i >> shift];
             VT sum = t[(size_t)i * NV + k];
+#pragma unroll 8
             for (uint32_t j = row[i]; j < row[i + 1]; ++j) {
                 const int64_t c = (int64_t)(base + (uint32_t)col[j]);
                 if (c >= lo && c < hi)
