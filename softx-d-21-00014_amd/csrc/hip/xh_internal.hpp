@@ -44,6 +44,9 @@ struct Mat {
     // for a column that is not the grid neighbour row + a*gx*gy + b*gx + d
     int32_t gx = 0, gy = 0, gz = 0;
     uint8_t *gcode = nullptr;
+    // core-block matrix of hybrid GS (entries inside each row's block of core_B rows only):
+    // core_B > 0, and core_emax = the largest entry count of one block
+    int64_t core_B = 0, core_emax = 0;
     size_t bytes = 0;            // device bytes owned
 };
 

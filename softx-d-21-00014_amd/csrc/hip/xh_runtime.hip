@@ -363,6 +363,9 @@ int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
             XH_HIP(hipMemcpy(v, vf.data(), nnz * 4, hipMemcpyHostToDevice));
     }
     M.inv_diag = A.inv_diag;
+    M.core_B = B;
+    for (int64_t lo = 0; lo < n; lo += B)
+        M.core_emax = std::max<int64_t>(M.core_emax, row[std::min(n, lo + B)] - row[lo]);
     M.bytes = acc;
     return 0;
 }

@@ -322,6 +322,66 @@ __global__ __launch_bounds__(64) void k_hsgs_lds(const uint32_t *__restrict__ ro
         x[(size_t)i * NV + k] = xs[(i - lo) * NV + k];
 }
 
+// The same sweeps on a core-block matrix (only entries inside the row's block, u32
+// columns): a group first stages its block's row pointers, local columns and values in
+// LDS with x, then sweeps from LDS; the next row's t and inverse diagonal are loaded one
+// row ahead.  Each lane reads and writes only its own RHS column of the staged x, so the
+// sweep needs no synchronisation.  Entry order and arithmetic as k_hsgs_lds.
+template <typename VT, int NV>
+__global__ __launch_bounds__(64) void k_hsgs_core(const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ col,
+                                                  const VT *__restrict__ val,
+                                                  const VT *__restrict__ invd, int64_t n,
+                                                  int64_t B, int64_t E, size_t gbytes,
+                                                  const VT *__restrict__ t,
+                                                  VT *__restrict__ x, int x_zero, VT relax,
+                                                  const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    extern __shared__ unsigned char smem[];
+    constexpr int G = 64 / NV;
+    const int g = threadIdx.x / NV, k = threadIdx.x % NV;
+    VT *xs = (VT *)(smem + (size_t)g * gbytes);
+    VT *cv = xs + B * NV;
+    uint32_t *cc = (uint32_t *)(cv + E);
+    uint32_t *rp = cc + E;
+    const int64_t lo = ((int64_t)blockIdx.x * G + g) * B;
+    const bool active = lo < n;
+    const int64_t hi = active ? (lo + B < n ? lo + B : n) : lo;
+    const int64_t nr = hi - lo;
+    const uint32_t e0 = active ? row[lo] : 0u;
+    const uint32_t ne = active ? row[hi] - e0 : 0u;
+    for (int64_t q = k; q <= nr && active; q += NV)
+        rp[q] = row[lo + q] - e0;
+    for (uint32_t q = k; q < ne; q += NV) {
+        cc[q] = col[e0 + q] - (uint32_t)lo;
+        cv[q] = val[e0 + q];
+    }
+    for (int64_t q = 0; q < nr; ++q)
+        xs[q * NV + k] = x_zero ? VT(0) : x[(size_t)(lo + q) * NV + k];
+    __syncthreads();
+    for (int sweep = 0; sweep < 2 && nr > 0; ++sweep) {
+        int64_t li = sweep == 0 ? 0 : nr - 1;
+        VT tn = t[(size_t)(lo + li) * NV + k], dn = invd[lo + li];
+        for (int64_t q = 0; q < nr; ++q) {
+            const int64_t lc = li;
+            VT sum = tn;
+            const VT di = dn;
+            if (q + 1 < nr) {
+                li = sweep == 0 ? q + 1 : nr - 2 - q;
+                tn = t[(size_t)(lo + li) * NV + k];
+                dn = invd[lo + li];
+            }
+            for (uint32_t j = rp[lc]; j < rp[lc + 1]; ++j)
+                sum = sum - xs[(size_t)cc[j] * NV + k] * cv[j];
+            const VT coeff = relax * di;
+            xs[lc * NV + k] = xs[lc * NV + k] + coeff * sum;
+        }
+    }
+    for (int64_t q = 0; q < nr; ++q)
+        x[(size_t)(lo + q) * NV + k] = xs[q * NV + k];
+}
+
 // same sweep order on x in global memory, for blocks that do not fit in LDS
 template <typename VT, typename IT, int NV>
 __global__ __launch_bounds__(64) void k_hsgs_glb(const uint32_t *__restrict__ row,
@@ -459,6 +519,18 @@ void hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int6
     const int64_t nblk = (A.nrows + B - 1) / B;
     const int grid = (int)((nblk + G - 1) / G);
     const size_t lds = (size_t)64 * B * sizeof(VT);
+    if constexpr (sizeof(IT) == 4) {
+        // a core-block matrix of this block size whose blocks fit in LDS: staged sweeps
+        const size_t gb = ((size_t)B * NV * sizeof(VT) + (size_t)A.core_emax * (sizeof(VT) + 4) +
+                           (size_t)(B + 1) * 4 + 15) / 16 * 16;
+        if (A.core_B == B && G * gb <= 64 * 1024) {
+            hipLaunchKernelGGL((k_hsgs_core<VT, NV>), dim3(grid), dim3(64), G * gb, s, A.row,
+                               (const uint32_t *)A.col, (const VT *)A.val,
+                               (const VT *)A.inv_diag, A.nrows, B, A.core_emax, gb,
+                               (const VT *)t, (VT *)x, x_zero, (VT)relax, done);
+            return;
+        }
+    }
     if (lds <= 64 * 1024) {
         hipLaunchKernelGGL((k_hsgs_lds<VT, IT, NV>), dim3(grid), dim3(64), lds, s, A.row,
                            (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
