@@ -767,9 +767,12 @@ def default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Cheb
     p = Params()
     p.solver = role(method, max_iters=max_iters, abs_tol=abs_tol, rel_tol=rel_tol,
                     conv_check=1)
-    p.precond = role(precond, max_iters=1) if precond else role(NONE)
+    # poly_order, relax and fraction apply to every role that uses them: the smoothers
+    # and a Chebyshev / Jacobi / HSGS preconditioner without MultiGrid
     sm = dict(max_iters=sweeps, relax_factor=relax, poly_order=poly_order,
               spectrum_fraction=fraction)
+    p.precond = (role(precond, max_iters=1, relax_factor=relax, poly_order=poly_order,
+                      spectrum_fraction=fraction) if precond else role(NONE))
     p.pre = role(smoother, **sm)
     p.post = role(smoother, **sm)
     p.coarse = role(coarse, **sm)
