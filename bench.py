@@ -159,9 +159,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X and this build has no "
-              "multi-GPU path (see DESIGN.md for how the solve would shard); run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the solve runs on one MI355X and has no multi-GPU "
+              "path (row-block sharding with a halo exchange per SpMV is a design only); run "
+              "with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import pyxamg as X
