@@ -138,8 +138,9 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
  * structured-grid SpMV of matrices with a grid view), "spmv_grid_items" (default 2048),
  * "spmv_grid_min_seg" (default 32), "unet_fuse_up" (0/1, default 1: U-Net inference reads
  * the decoder's upsampled inputs through the convolution instead of writing them),
- * "unet_conv_tile" (0/1, default 1: forward convolutions with 16 output channels on
- * volumes of w % 64 == h % 4 == d % 2 == 0 read their input from LDS-staged tiles). */
+ * "unet_conv_tile" (0/1, default 1: forward convolutions with 16 or 32 output channels
+ * on volumes of w % 64 == h % 4 == d % 2 == 0, and with 48 on w % 32 == 0, read their
+ * input from LDS-staged tiles). */
 int xh_set_option(const char *key, int64_t value);
 
 /* Compressed block: the reference's compact-index CSR block, as csr_matrix::compress

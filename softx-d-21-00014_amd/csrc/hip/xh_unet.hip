@@ -196,17 +196,18 @@ __global__ __launch_bounds__(256) void k_conv(Dims g, const bf16_t *__restrict__
 }
 
 // ---- forward convolution over LDS-staged tiles (cout = 16 * CT) ----
-// A workgroup computes kTZ x TY lines of kTX voxels.  The input those voxels read (the tile
-// with a one-voxel halo, or with UP the coarse voxels under it) is staged in LDS once, zero
-// outside the volume, each staged line stored as NCG groups of 8 channels (16 bytes) by
-// voxel, so the 16 rows of an input fragment are 256 contiguous bytes.  All 27 taps then
-// read their input fragments from LDS instead of gathering them through the L1 per tap.
-// With CT = 1 and at most 16 k steps the weight fragments of all steps are loaded after
-// the staging stores; otherwise each step's CT fragments are loaded one step ahead.  The weights are
-// the A operand (the MFMA gives the transposed output tile, 4 channels of a voxel per lane,
-// so the stores are 8-byte vectors); the products, their k order and the epilogue's
-// arithmetic are those of k_conv, so the output is bit-identical.  The launch requires
-// w % kTX == 0, h % TY == 0 and d % kTZ == 0.
+// A workgroup computes kTZ x TY lines of TX voxels (64, or 32 for narrower grids).  The
+// input those voxels read (the tile with a one-voxel halo, or with UP the coarse voxels
+// under it) is staged in LDS once, zero outside the volume, each staged line stored as NCG
+// groups of 8 channels (16 bytes) by voxel, so the 16 rows of an input fragment are 256
+// contiguous bytes.  All 27 taps then read their input fragments from LDS instead of
+// gathering them through the L1 per tap.  With CT = 1 and at most 16 k steps the weight
+// fragments of all steps are loaded after the staging stores; otherwise each step's CT
+// fragments are loaded one step ahead.  The weights are the A operand (the MFMA gives the
+// transposed output tile, 4 channels of a voxel per lane, so the stores are 8-byte
+// vectors); the products, their k order and the epilogue's arithmetic are those of
+// k_conv, so the output is bit-identical.  The launch requires w % TX == 0, h % TY == 0
+// and d % kTZ == 0.
 constexpr int kTX = 64, kTY = 4, kTZ = 2;
 
 // Epilogue of the tiled convolutions: lane (r, kg) of tile t, channel tile c holds
@@ -258,7 +259,7 @@ __device__ __forceinline__ void conv_tile_store(const Dims &g, int nb, int z0, i
     }
 }
 
-template <int NCG, int CT, int TY, bool UP>
+template <int NCG, int CT, int TY, bool UP, int TX = kTX>
 __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restrict__ in,
                                                    const bf16_t *__restrict__ wt, int kpad,
                                                    const float *__restrict__ bias,
@@ -267,10 +268,10 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     constexpr int CIN = 8 * NCG;
     constexpr int SZ = UP ? kTZ / 2 + 2 : kTZ + 2;
     constexpr int SY = UP ? TY / 2 + 2 : TY + 2;
-    constexpr int SX = UP ? kTX / 2 + 2 : kTX + 2;
+    constexpr int SX = UP ? TX / 2 + 2 : TX + 2;
     constexpr int NCH = SZ * SY * SX * NCG; // 16-byte chunks staged
     constexpr int PER = (NCH + 255) / 256;
-    constexpr int XT = kTX / 16;              // 16-voxel tiles per line
+    constexpr int XT = TX / 16;               // 16-voxel tiles per line
     constexpr int LPW = kTZ * TY / 4;         // lines per wave
     constexpr int NT = LPW * XT;              // tiles per wave
     constexpr int KS = (27 * CIN + 31) / 32; // k steps (kpad == 32 * KS)
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     static_assert(LPW >= 1 && (kTZ * TY) % 4 == 0, "tile lines must split over 4 waves");
     __shared__ uint4 sm[NCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ntx = g.w / kTX, nty = g.h / TY, ntz = g.d / kTZ;
+    const int ntx = g.w / TX, nty = g.h / TY, ntz = g.d / kTZ;
     int64_t bid = blockIdx.x;
     const int tx = (int)(bid % ntx);
     bid /= ntx;
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(Dims g, const bf16_t *__restr
     bid /= nty;
     const int tz = (int)(bid % ntz);
     const int nb = (int)(bid / ntz);
-    const int x0 = tx * kTX, y0 = ty * TY, z0 = tz * kTZ;
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * kTZ;
     const int r = lane & 15, kg = lane >> 4;
     // lane's weight row (output channel 16c + r of channel tile c at + 16c * kpad)
     const bf16_t *wrow = wt + (size_t)r * kpad + 8 * kg;
@@ -1136,13 +1137,13 @@ void launch_conv(const Dims &g, const bf16_t *in, int cin, const bf16_t *wt, int
 // 0: every forward convolution runs as k_conv (no LDS-staged tiles)
 int g_conv_tile = 1;
 
-template <int NCG, int CT, int TY, bool UP>
+template <int NCG, int CT, int TY, bool UP, int TX = kTX>
 void launch_conv_tile(const Dims &g, const bf16_t *in, const bf16_t *wt, int kpad,
                       const float *bias, const bf16_t *add, int relu, bf16_t *out,
                       hipStream_t s) {
-    const dim3 grid((unsigned)(g.vox() / (kTX * TY * kTZ))), blk(256);
-    hipLaunchKernelGGL((k_conv_tile<NCG, CT, TY, UP>), grid, blk, 0, s, g, in, wt, kpad, bias,
-                       add, relu, out);
+    const dim3 grid((unsigned)(g.vox() / (TX * TY * kTZ))), blk(256);
+    hipLaunchKernelGGL((k_conv_tile<NCG, CT, TY, UP, TX>), grid, blk, 0, s, g, in, wt, kpad,
+                       bias, add, relu, out);
 }
 
 // up: `in` is the coarse input of a nearest x2 upsampling onto g (fused)
@@ -1187,6 +1188,21 @@ void conv_fwd(xh_unet_s *U, const Conv &c, const Dims &g, const bf16_t *in, cons
         }
         if (c.cin == 48 && up) {
             launch_conv_tile<6, 2, 4, true>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+    }
+    // 48 output channels on lines of 32 voxels (the third level of a 128^3 grid)
+    if (g_conv_tile && c.cout == 48 && g.w % 32 == 0 && g.h % kTY == 0 && g.d % kTZ == 0) {
+        if (c.cin == 32 && !up) {
+            launch_conv_tile<4, 3, 4, false, 32>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 48 && !up) {
+            launch_conv_tile<6, 3, 2, false, 32>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
+            return;
+        }
+        if (c.cin == 64 && up) {
+            launch_conv_tile<8, 3, 4, true, 32>(g, in, c.wt, c.kpad, c.b, add, rl, out, st);
             return;
         }
     }

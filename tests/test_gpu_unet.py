@@ -133,15 +133,16 @@ def test_fused_upsampling_inference_is_identical(gpu):
 
 def test_tiled_convolution_is_identical(gpu):
     # forward convolutions with 16 or 32 output channels on volumes with w % 64 == h % 4 ==
-    # d % 2 == 0 read their input from LDS-staged tiles (plain, and through the fused
-    # upsampling); products, k order and epilogue are those of the per-tap gathers, so the
-    # result is bit-identical
+    # d % 2 == 0 (48 channels: w % 32 == 0) read their input from LDS-staged tiles (plain,
+    # and through the fused upsampling); products, k order and epilogue are those of the
+    # per-tap gathers, so the result is bit-identical
     X = gpu
     try:
         # level-1 tiles (32 output channels) on the 128 x 32 x 16 grid: 16 -> 32, 32 -> 32
-        # (2-line tiles) and the upsampled 48 -> 32 and 32 -> 32
+        # (2-line tiles) and the upsampled 48 -> 32 and 32 -> 32; level-2 tiles (48 output
+        # channels, lines of 32 voxels): 32 -> 48, 48 -> 48 and the upsampled 64 -> 48
         for ch, fuse in (((16, 32, 48), 1), ((16, 32, 48), 0), ((16, 16, 32), 1),
-                         ((16, 32, 32), 1)):
+                         ((16, 32, 32), 1), ((16, 32, 48, 64), 1)):
             net = X.UNet(ch, seed=5)
             net.output_scale = 0.5
             X.set_option("unet_fuse_up", fuse)
