@@ -35,7 +35,23 @@ typedef struct {
     int32_t agg_pmax;   /* mg_agg_Pmax_elements 4 */
     int32_t eig_iters;  /* power iterations for Chebyshev bounds (10) */
     uint64_t eig_seed;
+    int32_t coarsen_type;    /* mg_coarsening_type: 6 Falgout (Ruge-Stueben on one
+                                process, the default), 8 PMIS */
+    int32_t interp_type;     /* mg_interpolation_type: 0 classical (default), 6 extended+i */
+    int32_t agg_interp_type; /* mg_agg_interpolation_type: 4 multipass (default) */
+    /* keys of levels 0..nlev_keys-1 (the "key=value@levX-Y" overrides of a per-level
+     * hierarchy, src/hypre/detail/hypre_hierarchy.inl:54-108); other levels use the keys
+     * above.  lev_keys may be NULL when nlev_keys is 0. */
+    int32_t nlev_keys;
+    const struct xamg_amg_level_keys_s *lev_keys;
 } xamg_amg_params;
+
+typedef struct xamg_amg_level_keys_s {
+    float strength, max_row_sum, trunc_factor;
+    int32_t pmax, coarsen_type, interp_type, agg_interp_type;
+    float agg_trunc_factor;
+    int32_t agg_pmax, num_paths;
+} xamg_amg_level_keys;
 
 void xamg_amg_default_params(xamg_amg_params *p);
 int xamg_amg_setup(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
@@ -48,6 +64,28 @@ int xamg_amg_level(xamg_amg H, int level, int which, int64_t *nrows, int64_t *nc
 int xamg_amg_eigs(xamg_amg H, int level, double *min_eig, double *max_eig);
 /* re-estimate lambda_max of D^-1/2 A D^-1/2 on every level by `iters` power iterations */
 int xamg_amg_estimate_eigs(xamg_amg H, int iters, uint64_t seed);
+/* The steps of one coarsening level, for tests of what the algorithms fix (A square,
+ * n rows): strength flags per nonzero (1 strong), the C/F splitting (1 C, -1 F) of
+ * coarsen_type with the given PMIS seed, and the interpolation P (n x nc, interp_type
+ * 0, 4 or 6) of a splitting, returned like a level of a hierarchy (free with
+ * xamg_amg_free; which = 1 of level 0). */
+int xamg_amg_strength(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                      float theta, float max_row_sum, uint8_t *strong);
+int xamg_amg_split(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                   const uint8_t *strong, int coarsen_type, uint64_t seed, int8_t *cf);
+int xamg_amg_interp(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                    const uint8_t *strong, const int8_t *cf, int interp_type, float trunc,
+                    int pmax, xamg_amg *out);
+/* The parameters XAMG_solver_create would pass on for the XAMG_* parameter set `pid`
+ * (include/xamg_c_api.h): the solver-side roles with the per-level smoother roles of
+ * the "@levX-Y" overrides resolved (params.get("pre_smoother", l),
+ * src/solvers/detail/mg.inl:113-131), and the AMG setup keys, per level for a per-level
+ * hierarchy (src/hypre/detail/hypre_hierarchy.inl:54-108).  lev_keys (nullable) receives
+ * setup->nlev_keys entries, at most XH_MAX_LEVELS; setup->lev_keys is set to NULL.
+ * Returns what XAMG_solver_create would return for the parameters (e.g. -3 for an
+ * unsupported coarsening type). */
+int xamg_param_resolve(size_t pid, xh_params *prm, xamg_amg_params *setup,
+                       xamg_amg_level_keys *lev_keys);
 /* copy into a device-side hierarchy object of libxamg_hip */
 int xamg_amg_to_device(xamg_amg H, xh_hier *out);
 int xamg_amg_free(xamg_amg H);

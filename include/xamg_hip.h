@@ -177,6 +177,10 @@ int xh_eval_bumps(int nx, int ny, int nz, int nv, int max_bumps, const int32_t *
 /* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
 int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);
 
+/* most hierarchy levels with a role of their own (the reference's default
+ * mg_max_levels is 25, src/param/xamg_params_details.h:186) */
+#define XH_MAX_LEVELS 32
+
 /* One parameter role of the reference param dictionary
  * (src/param/xamg_params_details.h:79-236). */
 typedef struct {
@@ -205,6 +209,15 @@ typedef struct {
                                 counterpart (0, the default, is the reference behaviour);
                                 not for MergedBiCGStab, which updates x after r */
     int32_t reserved[3];
+    /* Per-level smoother roles: MultiGrid::setup builds the smoothers of level l from
+     * params.get("pre_smoother", l) and params.get("post_smoother", l), the role lists
+     * with the "key=value@levX-Y" overrides of that level applied (src/solvers/detail/
+     * mg.inl:113-131, api/c/xamg_c_api.cpp:256-260).  Levels l < nlev_roles use
+     * pre_lev[l] / post_lev[l]; the other levels use pre / post.  0: every level uses
+     * pre / post.  The coarse-grid solver has one role (params.get("coarse_grid_solver")
+     * takes no level). */
+    int32_t nlev_roles;
+    xh_role pre_lev[XH_MAX_LEVELS], post_lev[XH_MAX_LEVELS];
 } xh_params;
 
 /* Hierarchy (the output of hypre::get_full_hierarchy, hypre_hierarchy.inl:31-52):

@@ -21,6 +21,9 @@ class Csr(C.Structure):
                 ("row", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
 
 
+MAX_LEVELS = 32  # XO_MAX_LEVELS
+
+
 class Role(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_iters", C.c_int32), ("abs_tol", C.c_float),
                 ("rel_tol", C.c_float), ("conv_check", C.c_int32), ("relax_factor", C.c_float),
@@ -32,7 +35,8 @@ class Params(C.Structure):
                 ("coarse", Role), ("mg_cycle", C.c_int32), ("mg_prec", C.c_int32),
                 ("hsgs_ncores", C.c_int32), ("hsgs_block_rows", C.c_int32),
                 ("dot_ncores", C.c_int32), ("coarse_fp64", C.c_int32),
-                ("true_res_period", C.c_int32)]
+                ("true_res_period", C.c_int32), ("nlev_roles", C.c_int32),
+                ("pre_lev", Role * MAX_LEVELS), ("post_lev", Role * MAX_LEVELS)]
 
 
 class HierS(C.Structure):
@@ -210,6 +214,12 @@ def params_from(xp, dot_ncores=1):
     p.dot_ncores = dot_ncores
     p.coarse_fp64 = xp.coarse_fp64
     p.true_res_period = xp.true_res_period
+    p.nlev_roles = xp.nlev_roles
+    for l in range(xp.nlev_roles):
+        for f in ("pre_lev", "post_lev"):
+            r = getattr(xp, f)[l]
+            getattr(p, f)[l] = Role(r.method, r.max_iters, r.abs_tol, r.rel_tol, r.conv_check,
+                                    r.relax_factor, r.poly_order, r.spectrum_fraction)
     return p
 
 

@@ -35,6 +35,11 @@ namespace xo {
 
 static int g_err = 0; /* set on a condition where the reference asserts */
 
+/* Loops over rows or vector elements run on all OpenMP threads: each element is
+ * computed by one thread with the same operations, so results do not depend on the
+ * thread count.  Reductions keep their own fixed order (per-core partials, see dot). */
+#define XO_PAR _Pragma("omp parallel for schedule(static)")
+
 /* vector with the reference's zero flag; data layout x[i*nv+k]
  * (src/primitives/vector/vector.h:509-526) */
 template <typename T>
@@ -47,6 +52,14 @@ struct Vec {
         n = n_;
         nv = nv_;
         d.assign((size_t)n_ * nv_, T(0));
+        zero = true;
+    }
+    /* a work buffer kept between calls, as the reference keeps its solver buffers: its
+     * memory is reused and only the zero flag is set (set_const(buf, 0.0) without force,
+     * blas_wrapper.inl:30-42); nothing reads a zero-flagged vector's data */
+    void reuse(uint64_t n_, int nv_) {
+        if (n != n_ || nv != nv_ || d.size() != (size_t)n_ * nv_)
+            alloc(n_, nv_);
         zero = true;
     }
     T *p() { return d.data(); }
@@ -103,9 +116,11 @@ static void ax_y(const Vec<T> &a, const Vec<T> &x, Vec<T> &y) { /* blas.inl:31-8
     const int nv = x.nv;
     const size_t N = x.sz();
     if (a.nv == 1) {
+        XO_PAR
         for (size_t i = 0; i < N; ++i)
             y.d[i] = a.d[0] * x.d[i];
     } else {
+        XO_PAR
         for (size_t i = 0; i < N; i += nv)
             for (int k = 0; k < nv; ++k)
                 y.d[i + k] = a.d[k] * x.d[i + k];
@@ -122,19 +137,23 @@ static void scal(const Vec<T> &x, Vec<T> &y) { /* blas.inl:156-219 */
     const int nv = y.nv;
     if (x.nv == 1) {
         if (x.n == 1) {
+            XO_PAR
             for (size_t i = 0; i < y.sz(); ++i)
                 y.d[i] *= x.d[0];
         } else {
+            XO_PAR
             for (uint64_t i = 0; i < y.n; ++i)
                 for (int k = 0; k < nv; ++k)
                     y.d[i * nv + k] *= x.d[i];
         }
     } else {
         if (x.n == 1) {
+            XO_PAR
             for (size_t i = 0; i < y.sz(); i += nv)
                 for (int k = 0; k < nv; ++k)
                     y.d[i + k] *= x.d[k];
         } else {
+            XO_PAR
             for (size_t i = 0; i < y.sz(); ++i)
                 y.d[i] *= x.d[i];
         }
@@ -153,9 +172,11 @@ static void axpby(const Vec<T> &a, const Vec<T> &x, const Vec<T> &b, Vec<T> &y) 
     }
     const int nv = y.nv;
     if (a.nv == 1) {
+        XO_PAR
         for (size_t i = 0; i < y.sz(); ++i)
             y.d[i] = a.d[0] * x.d[i] + b.d[0] * y.d[i];
     } else {
+        XO_PAR
         for (size_t i = 0; i < y.sz(); i += nv)
             for (int k = 0; k < nv; ++k)
                 y.d[i + k] = a.d[k] * x.d[i + k] + b.d[k] * y.d[i + k];
@@ -204,15 +225,18 @@ static void dot(const Vec<T> &x, const Vec<T> &y, Vec<T> &res) { /* blas.inl:465
     /* per-core sequential partial sums combined in rank order
      * (src/comm/shm_allreduce.h:112-135); one core = one sequential sum */
     std::vector<uint64_t> off = core_ranges(x.n, g_dot_ncores);
-    std::vector<T> acc(nv, T(0));
+    std::vector<std::vector<T>> parts(g_dot_ncores, std::vector<T>(nv, T(0)));
+#pragma omp parallel for schedule(static, 1)
     for (int c = 0; c < g_dot_ncores; ++c) {
-        std::vector<T> part(nv, T(0));
+        std::vector<T> &part = parts[c];
         for (uint64_t i = off[c]; i < off[c + 1]; ++i)
             for (int k = 0; k < nv; ++k)
                 part[k] += x.d[i * nv + k] * y.d[i * nv + k];
-        for (int k = 0; k < nv; ++k)
-            acc[k] = (c == 0) ? part[k] : acc[k] + part[k];
     }
+    std::vector<T> acc(nv, T(0));
+    for (int c = 0; c < g_dot_ncores; ++c)
+        for (int k = 0; k < nv; ++k)
+            acc[k] = (c == 0) ? parts[c][k] : acc[k] + parts[c][k];
     for (int k = 0; k < nv; ++k)
         res.d[k] = acc[k];
     res.zero = false;
@@ -231,10 +255,12 @@ static void axpby_z(const Vec<T> &a, const Vec<T> &x, const Vec<T> &b, const Vec
     }
     const int nv = x.nv;
     if (a.nv == 1) {
+        XO_PAR
         for (size_t i = 0; i < x.sz(); i += nv)
             for (int k = 0; k < nv; ++k)
                 z.d[i + k] = a.d[0] * x.d[i + k] + b.d[0] * y.d[i + k];
     } else {
+        XO_PAR
         for (size_t i = 0; i < x.sz(); i += nv)
             for (int k = 0; k < nv; ++k)
                 z.d[i + k] = a.d[k] * x.d[i + k] + b.d[k] * y.d[i + k];
@@ -259,10 +285,12 @@ static void axpbypcz(const Vec<T> &a, const Vec<T> &x, const Vec<T> &b, const Ve
     }
     const int nv = x.nv;
     if (a.nv == 1) {
+        XO_PAR
         for (size_t i = 0; i < x.sz(); i += nv)
             for (int k = 0; k < nv; ++k)
                 z.d[i + k] = a.d[0] * x.d[i + k] + b.d[0] * y.d[i + k] + c.d[0] * z.d[i + k];
     } else {
+        XO_PAR
         for (size_t i = 0; i < x.sz(); i += nv)
             for (int k = 0; k < nv; ++k)
                 z.d[i + k] = a.d[k] * x.d[i + k] + b.d[k] * y.d[i + k] + c.d[k] * z.d[i + k];
@@ -280,6 +308,7 @@ static void xdivy_z(const Vec<T> &x, const Vec<T> &y, Vec<T> &z) { /* blas_ext.i
         g_err = -2; /* "Div by 0!" assert */
         return;
     }
+    XO_PAR
     for (size_t i = 0; i < x.sz(); ++i)
         z.d[i] = x.d[i] / y.d[i];
     z.zero = false;
@@ -295,9 +324,10 @@ static void group1(Vec<T> &r0, Vec<T> &b, Vec<T> &p, Vec<T> &r, Vec<T> &rho0) { 
         set_const(r0, 0.0, true);
     const int nv = r0.nv;
     std::vector<uint64_t> off = core_ranges(r0.n, g_dot_ncores);
-    std::vector<T> acc(nv, T(0));
+    std::vector<std::vector<T>> parts(g_dot_ncores, std::vector<T>(nv, T(0)));
+#pragma omp parallel for schedule(static, 1)
     for (int c = 0; c < g_dot_ncores; ++c) {
-        std::vector<T> part(nv, T(0));
+        std::vector<T> &part = parts[c];
         for (uint64_t i = off[c] * nv; i < off[c + 1] * nv; i += nv)
             for (int k = 0; k < nv; ++k) {
                 r0.d[i + k] = b.d[i + k] - r0.d[i + k];
@@ -305,9 +335,11 @@ static void group1(Vec<T> &r0, Vec<T> &b, Vec<T> &p, Vec<T> &r, Vec<T> &rho0) { 
                 p.d[i + k] = r0.d[i + k];
                 r.d[i + k] = r0.d[i + k];
             }
-        for (int k = 0; k < nv; ++k)
-            acc[k] = (c == 0) ? part[k] : acc[k] + part[k];
     }
+    std::vector<T> acc(nv, T(0));
+    for (int c = 0; c < g_dot_ncores; ++c)
+        for (int k = 0; k < nv; ++k)
+            acc[k] = (c == 0) ? parts[c][k] : acc[k] + parts[c][k];
     for (int k = 0; k < nv; ++k)
         rho0.d[k] = acc[k];
     r0.zero = rho0.zero = p.zero = r.zero = false;
@@ -321,15 +353,20 @@ static void group2(Vec<T> &t, Vec<T> &s, Vec<T> &phi, Vec<T> &psi, Vec<T> &theta
         set_const(s, 0.0, true);
     const int nv = s.nv;
     std::vector<uint64_t> off = core_ranges(s.n, g_dot_ncores);
-    std::vector<T> a1(nv, 0), a2(nv, 0), a3(nv, 0);
+    std::vector<std::vector<T>> parts(g_dot_ncores, std::vector<T>(3 * nv, T(0)));
+#pragma omp parallel for schedule(static, 1)
     for (int c = 0; c < g_dot_ncores; ++c) {
-        std::vector<T> p1(nv, 0), p2(nv, 0), p3(nv, 0);
+        T *p1 = parts[c].data(), *p2 = p1 + nv, *p3 = p2 + nv;
         for (uint64_t i = off[c] * nv; i < off[c + 1] * nv; i += nv)
             for (int k = 0; k < nv; ++k) {
                 p1[k] += t.d[i + k] * s.d[i + k];
                 p2[k] += t.d[i + k] * t.d[i + k];
                 p3[k] += s.d[i + k] * s.d[i + k];
             }
+    }
+    std::vector<T> a1(nv, 0), a2(nv, 0), a3(nv, 0);
+    for (int c = 0; c < g_dot_ncores; ++c) {
+        const T *p1 = parts[c].data(), *p2 = p1 + nv, *p3 = p2 + nv;
         for (int k = 0; k < nv; ++k) {
             a1[k] = (c == 0) ? p1[k] : a1[k] + p1[k];
             a2[k] = (c == 0) ? p2[k] : a2[k] + p2[k];
@@ -355,17 +392,20 @@ static void group4(Vec<T> &s, Vec<T> &t, Vec<T> &r0, Vec<T> &r, const Vec<T> &om
         set_const(r0, 0.0, true);
     const int nv = s.nv;
     std::vector<uint64_t> off = core_ranges(s.n, g_dot_ncores);
-    std::vector<T> acc(nv, 0);
+    std::vector<std::vector<T>> parts(g_dot_ncores, std::vector<T>(nv, T(0)));
+#pragma omp parallel for schedule(static, 1)
     for (int c = 0; c < g_dot_ncores; ++c) {
-        std::vector<T> part(nv, 0);
+        std::vector<T> &part = parts[c];
         for (uint64_t i = off[c] * nv; i < off[c + 1] * nv; i += nv)
             for (int k = 0; k < nv; ++k) {
                 r.d[i + k] = s.d[i + k] - omega.d[k] * t.d[i + k];
                 part[k] += r.d[i + k] * r0.d[i + k];
             }
-        for (int k = 0; k < nv; ++k)
-            acc[k] = (c == 0) ? part[k] : acc[k] + part[k];
     }
+    std::vector<T> acc(nv, 0);
+    for (int c = 0; c < g_dot_ncores; ++c)
+        for (int k = 0; k < nv; ++k)
+            acc[k] = (c == 0) ? parts[c][k] : acc[k] + parts[c][k];
     for (int k = 0; k < nv; ++k)
         rho.d[k] = acc[k];
     r.zero = rho.zero = false;
@@ -390,6 +430,7 @@ static void group5(Vec<T> &x, Vec<T> &p, Vec<T> &s, Vec<T> &r, Vec<T> &v, const 
     std::vector<T> val(nv);
     for (int k = 0; k < nv; ++k)
         val[k] = beta.d[k] * omega.d[k];
+    XO_PAR
     for (size_t i = 0; i < x.sz(); i += nv)
         for (int k = 0; k < nv; ++k) {
             x.d[i + k] += ac.d[k] * p.d[i + k] + oc.d[k] * s.d[i + k];
@@ -423,6 +464,7 @@ static Csr<T> make_csr(const xo_csr *m) {
 
 template <typename T>
 static void csr_ax_y(const Csr<T> &m, const T *x, T *y, int nv) { /* csr_matvec.inl:213-231 */
+    XO_PAR
     for (uint64_t i = 0; i < m.nrows; ++i) {
         T *yr = y + i * nv;
         for (int k = 0; k < nv; ++k)
@@ -437,6 +479,7 @@ static void csr_ax_y(const Csr<T> &m, const T *x, T *y, int nv) { /* csr_matvec.
 
 template <typename T>
 static void csr_axpy(const Csr<T> &m, const T *x, T *y, int nv) { /* csr_matvec.inl:103-116 */
+    XO_PAR
     for (uint64_t i = 0; i < m.nrows; ++i) {
         T *yr = y + i * nv;
         for (uint32_t j = m.row[i]; j < m.row[i + 1]; ++j) {
@@ -493,6 +536,7 @@ static std::vector<T> inv_sqrt_diag(const Csr<T> &m) { /* csr_matrix.h:336-368 *
 template <typename T>
 static void dense_ax_y(uint64_t n, uint64_t m, const T *D, const T *x, T *y, int nv) {
     /* dense_matvec.inl:109-117 */
+    XO_PAR
     for (uint64_t i = 0; i < n; ++i) {
         for (int k = 0; k < nv; ++k)
             y[i * nv + k] = 0.0;
@@ -727,8 +771,7 @@ struct Jacobi : Solver<T> { /* jacobi.inl:50-120 */
         Vec<T> relax;
         relax.alloc(1, nv);
         set_const(relax, (double)(T)this->prm.relax);
-        r.alloc(x.n, nv);
-        set_const(r, 0.0);
+        r.reuse(x.n, nv);
         auto &it = this->stats.iters;
         if (this->prm.conv_check) {
             Vec<T> rho0, res;
@@ -758,6 +801,7 @@ struct Jacobi : Solver<T> { /* jacobi.inl:50-120 */
         /* blas_merged_jacobi.inl:33-90 */
         const int nv = x.nv;
         if (x.zero && rr.zero) {
+            XO_PAR
             for (uint64_t i = 0; i < x.n; ++i)
                 for (int k = 0; k < nv; ++k)
                     x.d[i * nv + k] = relax.d[k] * invd.d[i] * b.d[i * nv + k];
@@ -766,6 +810,7 @@ struct Jacobi : Solver<T> { /* jacobi.inl:50-120 */
                 set_const(x, 0.0, true);
             if (rr.zero)
                 set_const(rr, 0.0, true);
+            XO_PAR
             for (uint64_t i = 0; i < x.n; ++i)
                 for (int k = 0; k < nv; ++k)
                     x.d[i * nv + k] +=
@@ -802,6 +847,7 @@ struct HSGS : Solver<T> { /* hsgs.inl:31-115, blas2.h:185-281, csr_gs.inl:33-134
          * ascending column is the stored order (csr_matrix_segment.inl:107-229) */
         set_const(t, 0.0, true);
         if (!x.zero) {
+            XO_PAR
             for (uint64_t i = 0; i < A->nrows; ++i)
                 for (uint32_t j = A->row[i]; j < A->row[i + 1]; ++j)
                     if (!inblk[j])
@@ -811,8 +857,11 @@ struct HSGS : Solver<T> { /* hsgs.inl:31-115, blas2.h:185-281, csr_gs.inl:33-134
         axpby_s(1.0, b, -1.0, t);
         if (x.zero)
             set_const(x, 0.0, true);
-        std::vector<T> sum(nv), coeff(nv);
-        for (size_t c = 0; c + 1 < blk.size(); ++c) {
+        /* blocks are independent (each reads and writes only its own rows of x) */
+        const int64_t nblk = (int64_t)blk.size() - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t c = 0; c < nblk; ++c) {
+            std::vector<T> sum(nv), coeff(nv);
             const uint64_t r0 = blk[c], r1 = blk[c + 1];
             for (uint64_t i = r0; i < r1; ++i) { /* forward, csr_gs.inl:77-100 */
                 for (int k = 0; k < nv; ++k)
@@ -855,8 +904,8 @@ struct HSGS : Solver<T> { /* hsgs.inl:31-115, blas2.h:185-281, csr_gs.inl:33-134
         Vec<T> relax;
         relax.alloc(1, nv);
         set_const(relax, (double)(T)this->prm.relax);
-        r.alloc(x.n, nv);
-        t.alloc(x.n, nv);
+        r.reuse(x.n, nv);
+        t.reuse(x.n, nv);
         auto &it = this->stats.iters;
         Vec<T> rho0, res;
         rho0.alloc(1, nv);
@@ -899,9 +948,9 @@ struct Chebyshev : Solver<T> { /* chebyshev.inl:124-206 */
         this->reset();
         const int order = this->prm.poly_order;
         Vec<T> a1 = const_vec<T>(nv, 1.0);
-        r.alloc(x.n, nv);
-        t1.alloc(x.n, nv);
-        t2.alloc(x.n, nv);
+        r.reuse(x.n, nv);
+        t1.reuse(x.n, nv);
+        t2.reuse(x.n, nv);
         std::vector<double> cd;
         if (cheby_coeffs(min_eig, max_eig, order, (double)this->prm.fraction, cd)) {
             g_err = -3;
@@ -973,6 +1022,10 @@ struct LevelData {
 
 struct Params {
     Role solver, precond, pre, post, coarse;
+    /* per-level smoother roles: params.get("pre_smoother", l) (mg.inl:113-131) */
+    std::vector<Role> pre_lev, post_lev;
+    const Role &pre_of(int l) const { return l < (int)pre_lev.size() ? pre_lev[l] : pre; }
+    const Role &post_of(int l) const { return l < (int)post_lev.size() ? post_lev[l] : post; }
     int mg_cycle = 0;
     int mg_prec = 0;
     int hsgs_ncores = 1, hsgs_block_rows = 0;
@@ -1029,6 +1082,8 @@ struct MultiGrid : Solver<double> {
     std::vector<AnySolver> pre, post;
     AnySolver coarse;
     bool coarse_as_fp64_in_f32 = false;
+    Vec<double> r;      /* residual buffer of solve() */
+    Vec<float> xf, bf;  /* fp32 x and b of the fp32 preconditioner (mg_prec 2) */
 
     void setup(const xo_hier *H, const Params &p) { /* mg.inl:97-164 */
         P = p;
@@ -1059,11 +1114,15 @@ struct MultiGrid : Solver<double> {
         post.resize(nl > 0 ? nl - 1 : 0);
         for (int l = 0; l + 1 < nl; ++l) {
             if (L[l].reduced) {
-                pre[l].f = make_smoother<float>(p.pre, &L[l].Af, L[l].min_eig, L[l].max_eig, p);
-                post[l].f = make_smoother<float>(p.post, &L[l].Af, L[l].min_eig, L[l].max_eig, p);
+                pre[l].f = make_smoother<float>(p.pre_of(l), &L[l].Af, L[l].min_eig,
+                                                L[l].max_eig, p);
+                post[l].f = make_smoother<float>(p.post_of(l), &L[l].Af, L[l].min_eig,
+                                                 L[l].max_eig, p);
             } else {
-                pre[l].d = make_smoother<double>(p.pre, &L[l].Ad, L[l].min_eig, L[l].max_eig, p);
-                post[l].d = make_smoother<double>(p.post, &L[l].Ad, L[l].min_eig, L[l].max_eig, p);
+                pre[l].d = make_smoother<double>(p.pre_of(l), &L[l].Ad, L[l].min_eig,
+                                                 L[l].max_eig, p);
+                post[l].d = make_smoother<double>(p.post_of(l), &L[l].Ad, L[l].min_eig,
+                                                  L[l].max_eig, p);
             }
         }
         LevelData &c = L[nl - 1];
@@ -1116,11 +1175,13 @@ struct MultiGrid : Solver<double> {
             bd.alloc(b.n, b.nv);
             xd.alloc(x.n, x.nv);
             bd.zero = b.zero;
+            XO_PAR
             for (size_t i = 0; i < b.sz(); ++i)
                 bd.d[i] = (double)b.d[i];
             Vec<double> ic = const_vec<double>(b.nv, 1.0);
             coarse.d->solve(xd, bd, ic);
             x.zero = xd.zero;
+            XO_PAR
             for (size_t i = 0; i < x.sz(); ++i)
                 x.d[i] = (T)xd.d[i];
             return;
@@ -1153,6 +1214,7 @@ struct MultiGrid : Solver<double> {
             Vec<T2> &temp32 = buf_tmp<T2>(d);
             temp32.zero = temp.zero;
             if (!temp.zero)
+                XO_PAR
                 for (size_t i = 0; i < temp.sz(); ++i)
                     temp32.d[i] = (T2)temp.d[i];
             Ax_y(matR<T2>(d), temp32, b_H);
@@ -1176,6 +1238,7 @@ struct MultiGrid : Solver<double> {
             Ax_y(matP<T2>(d), x_H, temp32);
             temp.zero = temp32.zero;
             if (!temp32.zero)
+                XO_PAR
                 for (size_t i = 0; i < temp.sz(); ++i)
                     temp.d[i] = (T1)temp32.d[i];
             axpby(a1, temp, a1, x);
@@ -1200,10 +1263,10 @@ struct MultiGrid : Solver<double> {
         alloc(nv);
         Vec<double> iconv = iconv_in;
         auto &it = this->stats.iters;
-        Vec<double> rho0, res, r;
+        Vec<double> rho0, res;
         rho0.alloc(1, nv);
         res.alloc(1, nv);
-        r.alloc(x.n, nv);
+        r.reuse(x.n, nv);
         if (this->prm.conv_check) {
             if (P.mg_prec == 2) {
                 g_err = -5; /* level-0 fp64 operator is not kept in that mode */
@@ -1220,24 +1283,27 @@ struct MultiGrid : Solver<double> {
             } else if (P.mg_prec == 2) {
                 /* fp32 preconditioner including level 0 (beyond the reference): b and x are
                  * converted at the preconditioner boundary */
-                Vec<float> xf, bf, icf;
-                xf.alloc(x.n, nv);
-                bf.alloc(b.n, nv);
+                Vec<float> icf;
+                xf.reuse(x.n, nv);
+                bf.reuse(b.n, nv);
                 icf.alloc(1, nv);
                 for (int k = 0; k < nv; ++k)
                     icf.d[k] = (float)iconv.d[k];
                 icf.zero = iconv.zero;
                 bf.zero = b.zero;
                 if (!b.zero)
+                    XO_PAR
                     for (size_t i = 0; i < b.sz(); ++i)
                         bf.d[i] = (float)b.d[i];
                 xf.zero = x.zero;
                 if (!x.zero)
+                    XO_PAR
                     for (size_t i = 0; i < x.sz(); ++i)
                         xf.d[i] = (float)x.d[i];
                 cycle<float, float>(xf, bf, 0, P.mg_cycle, icf);
                 x.zero = xf.zero;
                 if (!xf.zero)
+                    XO_PAR
                     for (size_t i = 0; i < x.sz(); ++i)
                         x.d[i] = (double)xf.d[i];
             } else if (L[1].reduced) {
@@ -1693,6 +1759,10 @@ static Params params_of(const xo_params *p) {
     P.pre = role_of(p->pre);
     P.post = role_of(p->post);
     P.coarse = role_of(p->coarse);
+    for (int l = 0; l < std::min(std::max(0, p->nlev_roles), XO_MAX_LEVELS); ++l) {
+        P.pre_lev.push_back(role_of(p->pre_lev[l]));
+        P.post_lev.push_back(role_of(p->post_lev[l]));
+    }
     P.mg_cycle = p->mg_cycle;
     P.mg_prec = p->mg_prec;
     P.hsgs_ncores = std::max(1, p->hsgs_ncores);

@@ -1,8 +1,11 @@
 /*
  * XAMG solve-phase oracle -- TEST INFRASTRUCTURE ONLY.
  *
- * A single-threaded CPU restatement of the reference XAMG solve phase, used as the
- * checker for the HIP implementation.  Only tests/, __graft_entry__.smoke() and the
+ * A CPU restatement of the reference XAMG solve phase, used as the checker for the HIP
+ * implementation.  Loops over rows and vector elements run on OpenMP threads (each element
+ * is computed by one thread, so results do not depend on the thread count); every
+ * reduction keeps the reference's order: sequential partial sums per emulated core
+ * (dot_ncores, split_range blocks), added in core order.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it, and only as the checker (or as the timed
  * CPU baseline), never as the thing that is shipped or measured on the GPU.
  *
@@ -48,6 +51,8 @@ typedef struct {
     const double *val;   /* nnz; cast to float on reduced-precision levels */
 } xo_csr;
 
+#define XO_MAX_LEVELS 32
+
 /* one parameter role (solver / preconditioner / pre_smoother / post_smoother /
  * coarse_grid_solver), src/param/xamg_params_details.h:79-236 */
 typedef struct {
@@ -71,6 +76,10 @@ typedef struct {
     int32_t coarse_fp64;     /* mg_prec==2: coarsest direct solve kept in fp64 */
     int32_t true_res_period; /* >0: every k-th iteration r = b - A x, rho = (r, r0)
                                 (no reference counterpart; 0 = reference behaviour) */
+    /* per-level smoother roles: level l < nlev_roles smooths with pre_lev[l] /
+     * post_lev[l] (params.get("pre_smoother", l), src/solvers/detail/mg.inl:113-131) */
+    int32_t nlev_roles;
+    xo_role pre_lev[XO_MAX_LEVELS], post_lev[XO_MAX_LEVELS];
 } xo_params;
 
 typedef struct {

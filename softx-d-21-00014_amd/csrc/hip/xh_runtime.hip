@@ -496,6 +496,9 @@ struct Level {
     Mat A, P, R;
     bool own_A = true;
     double mn = 0, mx = 0;
+    // the smoother roles of this level: params.get("pre_smoother", l) and
+    // params.get("post_smoother", l) (mg.inl:113-131)
+    xh_role pre{}, post{};
     int64_t hsgs_B = 0;
     // the entries of A inside each row's core block of hsgs_B rows (stored order, u32
     // columns; inv_diag is A's): what the hybrid GS sweeps read, built when a role of
@@ -652,7 +655,7 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
     hipStream_t st = S->stream;
     const int nl = (int)S->lv.size();
     if (cyc != 2) {
-        smooth(S, S->prm.pre, L, L.cheb_pre, x, b, x_zero);
+        smooth(S, L.pre, L, L.cheb_pre, x, b, x_zero);
         x_zero = false;
     }
     void *rt = (L.tprec == L.prec) ? L.tmp : L.tmp_t;
@@ -684,7 +687,7 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
         a.x_zero = x_zero ? 1 : 0;
         spmv(L.P, nv, H.x, x, OP_PROLONG32, a, st);
     }
-    smooth(S, S->prm.post, L, L.cheb_post, x, b, false);
+    smooth(S, L.post, L, L.cheb_post, x, b, false);
 }
 
 // preconditioner application x = M^{-1} b, x zero-flagged on entry
@@ -1317,9 +1320,22 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
         return true;
     };
     const bool mg = is_preconditioned(m) && prm->precond.method == XH_MULTIGRID;
-    if (mg && (!check_role(prm->pre, true) || !check_role(prm->post, true) ||
-               !check_role(prm->coarse, true)))
-        return fail(XH_ENOTSUP, "xh_solver_create: unsupported smoother / coarse solver");
+    if (prm->nlev_roles < 0 || prm->nlev_roles > XH_MAX_LEVELS)
+        return fail(XH_EINVAL, "xh_solver_create: nlev_roles must be in [0, XH_MAX_LEVELS]");
+    // the smoother roles of level l
+    auto pre_of = [&](int l) -> const xh_role & {
+        return l < prm->nlev_roles ? prm->pre_lev[l] : prm->pre;
+    };
+    auto post_of = [&](int l) -> const xh_role & {
+        return l < prm->nlev_roles ? prm->post_lev[l] : prm->post;
+    };
+    if (mg && !check_role(prm->coarse, true))
+        return fail(XH_ENOTSUP, "xh_solver_create: unsupported coarse solver");
+    if (mg)
+        for (int l = 0; l + 1 < nl; ++l)
+            if (!check_role(pre_of(l), true) || !check_role(post_of(l), true))
+                return fail(XH_ENOTSUP, "xh_solver_create: unsupported smoother on level " +
+                                            std::to_string(l));
     if (is_preconditioned(m) && !mg && !check_role(prm->precond, true))
         return fail(XH_ENOTSUP, "xh_solver_create: unsupported preconditioner");
     if (prm->mg_prec < 0 || prm->mg_prec > 2)
@@ -1379,7 +1395,9 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
             return bail(rc);
         // Chebyshev coefficients per role (the smoother objects of MultiGrid::setup,
         // mg.inl:113-131, each evaluate them from the level's eigenvalue estimates)
-        const xh_role *roles[3] = {mg ? &prm->pre : &prm->precond, &prm->post, &prm->coarse};
+        L.pre = mg ? pre_of(l) : prm->precond;
+        L.post = mg ? post_of(l) : prm->post;
+        const xh_role *roles[3] = {&L.pre, &L.post, &prm->coarse};
         std::vector<double> *dst[3] = {&L.cheb_pre, &L.cheb_post, &L.cheb_coarse};
         for (int q = 0; q < 3; ++q)
             if (roles[q]->method == XH_CHEBYSHEV &&

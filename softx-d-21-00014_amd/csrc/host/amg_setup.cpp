@@ -5,13 +5,19 @@
 // algorithm with the reference's default keys (src/param/xamg_params_details.h:179-209):
 //   strength      -a_ij >= theta * max_k(-a_ik) (sign relative to the diagonal),
 //                 rows whose |row sum| > max_row_sum * |a_ii| have no strong links
-//   coarsening    Ruge-Stueben first pass (measure = number of points that strongly
-//                 depend on a point), optional second pass; on the first
-//                 mg_agg_num_levels levels aggressive coarsening: a second Ruge-Stueben
-//                 pass over the C points with the long-range strength graph (at least
-//                 mg_num_paths paths of length <= 2)
-//   interpolation modified classical interpolation (hypre interp type 0); multipass
-//                 interpolation (hypre agg_interp_type 4) on aggressive levels
+//   coarsening    mg_coarsening_type 6 (Falgout; on one process its Ruge-Stueben
+//                 part): first pass (measure = number of points that strongly depend on
+//                 a point), optional second pass; 8 (PMIS, De Sterck, Yang and Heys, SIAM
+//                 J. Matrix Anal. Appl. 27 (2006)): measure + a random tie-break in (0,1),
+//                 a point whose measure beats every undecided strong neighbour's becomes
+//                 C, the undecided points that strongly depend on a new C point become F,
+//                 until every point is decided.  On the first mg_agg_num_levels levels
+//                 aggressive coarsening: a second pass of the same scheme over the C
+//                 points with the long-range strength graph (at least mg_num_paths paths
+//                 of length <= 2)
+//   interpolation mg_interpolation_type 0: modified classical; 6: extended+i (De Sterck,
+//                 Falgout, Nolting and Yang, Numer. Linear Algebra Appl. 15 (2008));
+//                 mg_agg_interpolation_type 4: multipass on aggressive levels
 //   truncation    drop |w| < trunc_factor * max|w|, keep at most Pmax entries,
 //                 rescale each row to its original sum
 //   Galerkin      A_c = R A P with R = P^T (hypre_base.inl:400-401)
@@ -296,7 +302,8 @@ std::vector<int8_t> rs_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, 
 
 // modified classical interpolation + truncation
 CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
-                      const std::vector<int8_t> &cf, const SetupParams &p, int64_t &nc) {
+                      const std::vector<int8_t> &cf, float trunc_factor, int pmax,
+                      int64_t &nc) {
     const int64_t n = A.nrows;
     std::vector<int64_t> cidx(n, -1);
     nc = 0;
@@ -366,10 +373,10 @@ CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
                 sum0 += e.second;
                 mx = std::max(mx, std::fabs(e.second));
             }
-            if (p.trunc_factor > 0.0f) {
+            if (trunc_factor > 0.0f) {
                 std::vector<std::pair<uint32_t, double>> kept;
                 for (auto &e : row)
-                    if (std::fabs(e.second) >= (double)p.trunc_factor * mx)
+                    if (std::fabs(e.second) >= (double)trunc_factor * mx)
                         kept.push_back(e);
                 double s1 = 0.0;
                 for (auto &e : kept)
@@ -379,7 +386,7 @@ CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
                         e.second *= sum0 / s1;
                 row.swap(kept);
             }
-            if (p.pmax > 0 && (int)row.size() > p.pmax) {
+            if (pmax > 0 && (int)row.size() > pmax) {
                 double s0 = 0.0;
                 for (auto &e : row)
                     s0 += e.second;
@@ -388,7 +395,7 @@ CsrHost interpolation(const CsrHost &A, const std::vector<uint8_t> &S,
                                     const std::pair<uint32_t, double> &b) {
                                      return std::fabs(a.second) > std::fabs(b.second);
                                  });
-                row.resize(p.pmax);
+                row.resize(pmax);
                 double s1 = 0.0;
                 for (auto &e : row)
                     s1 += e.second;
@@ -640,7 +647,249 @@ uint64_t splitmix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// PMIS (parallel modified independent set).  Measure w_i = |S^T_i| + u_i with u_i in
+// (0, 1) from a counter-based generator (equal measures, which the generator makes
+// vanishingly rare, are ordered by index), so the result does not depend on the thread
+// count.  A point that no point depends on becomes F (with no strong
+// dependency it gets an empty row of P, as an isolated point of the RS pass; with
+// isolated_as_c it becomes C, as in the second stage of aggressive coarsening).  Each
+// round, every undecided point whose measure exceeds that of all its undecided strong
+// neighbours (S_i and S^T_i) becomes C, then every undecided point that strongly depends
+// on one of the new C points becomes F.  Rounds see the state at their start, as the
+// parallel algorithm does.
+std::vector<int8_t> pmis_coarsen(const CsrHost &A, const std::vector<uint8_t> &S, uint64_t seed,
+                                 bool isolated_as_c = false) {
+    const int64_t n = A.nrows;
+    std::vector<uint32_t> tr(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+            if (S[j] && (int64_t)A.col[j] != i)
+                tr[A.col[j] + 1]++;
+    for (int64_t i = 0; i < n; ++i)
+        tr[i + 1] += tr[i];
+    std::vector<uint32_t> tc(tr[n]);
+    {
+        std::vector<uint32_t> pos(tr.begin(), tr.end() - 1);
+        for (int64_t i = 0; i < n; ++i)
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+                if (S[j] && (int64_t)A.col[j] != i)
+                    tc[pos[A.col[j]]++] = (uint32_t)i;
+    }
+    std::vector<double> w(n);
+    std::vector<int8_t> cf(n, UNDECIDED);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t z = splitmix64(seed * 0xD1B54A32D192ED03ull + (uint64_t)i);
+        const double u = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const int64_t deg = tr[i + 1] - tr[i];
+        w[i] = (double)deg + u;
+        bool dep = false;
+        for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j)
+            dep |= (S[j] != 0 && (int64_t)A.col[j] != i);
+        if (deg == 0)
+            cf[i] = (!dep && isolated_as_c) ? CPT : FPT;
+    }
+    std::vector<int64_t> und;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == UNDECIDED)
+            und.push_back(i);
+    std::vector<uint8_t> newc(n, 0);
+    while (!und.empty()) {
+        std::vector<int64_t> cs;
+        for (int64_t i : und) {
+            // (w, index) order: equal measures cannot stall the selection
+            auto beats = [&](int64_t k) { return w[k] > w[i] || (w[k] == w[i] && k > i); };
+            bool best = true;
+            for (uint32_t j = A.row[i]; best && j < A.row[i + 1]; ++j) {
+                const int64_t k = A.col[j];
+                if (S[j] && k != i && cf[k] == UNDECIDED && beats(k))
+                    best = false;
+            }
+            for (uint32_t q = tr[i]; best && q < tr[i + 1]; ++q) {
+                const int64_t k = tc[q];
+                if (cf[k] == UNDECIDED && beats(k))
+                    best = false;
+            }
+            if (best)
+                cs.push_back(i);
+        }
+        for (int64_t i : cs)
+            cf[i] = CPT;
+        for (int64_t c : cs)
+            for (uint32_t q = tr[c]; q < tr[c + 1]; ++q)
+                if (cf[tc[q]] == UNDECIDED)
+                    cf[tc[q]] = FPT;
+        std::vector<int64_t> rest;
+        for (int64_t i : und)
+            if (cf[i] == UNDECIDED)
+                rest.push_back(i);
+        und.swap(rest);
+    }
+    return cf;
+}
+
+// Extended+i interpolation.  For an F point i with strong C neighbours C_i and strong F
+// neighbours F_i, the interpolatory set is C^_i = C_i plus the strong C neighbours of
+// every k in F_i.  With abar_kl = a_kl when a_kl and a_kk have opposite signs (else 0)
+// and D_k = sum over l in C^_i + {i} of abar_kl:
+//   w_ij = -(a_ij + sum_{k in F_i} a_ik abar_kj / D_k) / d_i,   j in C^_i
+//   d_i  = a_ii + (sum of a_in over the other neighbours n, weak ones outside C^_i)
+//               + sum_{k in F_i} a_ik abar_ki / D_k,
+// a neighbour k with D_k = 0 lumped into d_i.  A zero-row-sum row interpolates constants
+// exactly (the weights sum to 1); truncation then rescales each row to its sum.
+CsrHost ext_i(const CsrHost &A, const std::vector<uint8_t> &S, const std::vector<int8_t> &cf,
+              float trunc, int pmax, int64_t &nc) {
+    const int64_t n = A.nrows;
+    std::vector<int64_t> cidx(n, -1);
+    nc = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (cf[i] == CPT)
+            cidx[i] = nc++;
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows(n);
+#pragma omp parallel
+    {
+        std::vector<int64_t> mark(n, -1);
+        std::vector<double> w(n, 0.0);
+#pragma omp for schedule(dynamic, 512)
+        for (int64_t i = 0; i < n; ++i) {
+            if (cf[i] == CPT) {
+                rows[i] = {{(uint32_t)cidx[i], 1.0}};
+                continue;
+            }
+            std::vector<uint32_t> chat;
+            auto add_c = [&](uint32_t c) {
+                if (mark[c] != i) {
+                    mark[c] = i;
+                    w[c] = 0.0;
+                    chat.push_back(c);
+                }
+            };
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+                const uint32_t k = A.col[j];
+                if (k == (uint32_t)i || !S[j])
+                    continue;
+                if (cf[k] == CPT)
+                    add_c(k);
+                else
+                    for (uint32_t q = A.row[k]; q < A.row[k + 1]; ++q)
+                        if (S[q] && A.col[q] != k && cf[A.col[q]] == CPT)
+                            add_c(A.col[q]);
+            }
+            if (chat.empty())
+                continue; // no interpolatory point: empty row of P
+            double d = 0.0;
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+                const uint32_t k = A.col[j];
+                if (k == (uint32_t)i)
+                    d += A.val[j];
+                else if (mark[k] == i)
+                    w[k] += A.val[j];
+                else if (!(S[j] && cf[k] != CPT))
+                    d += A.val[j];
+            }
+            for (uint32_t j = A.row[i]; j < A.row[i + 1]; ++j) {
+                const uint32_t k = A.col[j];
+                if (k == (uint32_t)i || !S[j] || cf[k] == CPT)
+                    continue;
+                const double akk = diag_of(A, k);
+                double D = 0.0, aki = 0.0;
+                for (uint32_t q = A.row[k]; q < A.row[k + 1]; ++q) {
+                    const uint32_t l = A.col[q];
+                    if (l == k || A.val[q] * akk >= 0.0)
+                        continue;
+                    if (l == (uint32_t)i) {
+                        D += A.val[q];
+                        aki += A.val[q];
+                    } else if (mark[l] == i) {
+                        D += A.val[q];
+                    }
+                }
+                if (D == 0.0) {
+                    d += A.val[j];
+                    continue;
+                }
+                const double f = A.val[j] / D;
+                for (uint32_t q = A.row[k]; q < A.row[k + 1]; ++q) {
+                    const uint32_t l = A.col[q];
+                    if (l != k && l != (uint32_t)i && mark[l] == i && A.val[q] * akk < 0.0)
+                        w[l] += f * A.val[q];
+                }
+                d += f * aki;
+            }
+            std::vector<std::pair<uint32_t, double>> row;
+            for (uint32_t c : chat)
+                row.emplace_back(c, d != 0.0 ? -w[c] / d : 0.0);
+            truncate_row(row, trunc, pmax);
+            for (auto &e : row)
+                e.first = (uint32_t)cidx[e.first];
+            std::sort(row.begin(), row.end());
+            rows[i] = std::move(row);
+        }
+    }
+    CsrHost P;
+    P.nrows = n;
+    P.ncols = nc;
+    P.row.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+        P.row[i + 1] = P.row[i] + (uint32_t)rows[i].size();
+    P.col.resize(P.row[n]);
+    P.val.resize(P.row[n]);
+    for (int64_t i = 0; i < n; ++i)
+        for (size_t q = 0; q < rows[i].size(); ++q) {
+            P.col[P.row[i] + q] = rows[i][q].first;
+            P.val[P.row[i] + q] = rows[i][q].second;
+        }
+    return P;
+}
+
+CsrHost interpolate_nc(const CsrHost &A, const std::vector<uint8_t> &S,
+                       const std::vector<int8_t> &cf, int type, float trunc, int pmax,
+                       int64_t &nc) {
+    if (type == INTERP_EXT_I)
+        return ext_i(A, S, cf, trunc, pmax, nc);
+    if (type == INTERP_MULTIPASS)
+        return multipass(A, S, cf, trunc, pmax, nc);
+    return interpolation(A, S, cf, trunc, pmax, nc);
+}
+
 } // namespace
+
+LevelKeys SetupParams::keys(int level) const {
+    if (level >= 0 && level < (int)per_level.size())
+        return per_level[level];
+    LevelKeys k;
+    k.strength = strength;
+    k.max_row_sum = max_row_sum;
+    k.trunc_factor = trunc_factor;
+    k.pmax = pmax;
+    k.coarsen_type = coarsen_type;
+    k.interp_type = interp_type;
+    k.agg_interp_type = agg_interp_type;
+    k.agg_trunc_factor = agg_trunc_factor;
+    k.agg_pmax = agg_pmax;
+    k.num_paths = num_paths;
+    return k;
+}
+
+bool coarsen_supported(int type) { return type == COARSEN_FALGOUT || type == COARSEN_PMIS; }
+bool interp_supported(int type) { return type == INTERP_CLASSICAL || type == INTERP_EXT_I; }
+bool agg_interp_supported(int type) { return type == INTERP_MULTIPASS; }
+
+std::vector<uint8_t> strength_of(const CsrHost &A, float theta, float max_row_sum) {
+    return strength(A, theta, max_row_sum);
+}
+
+std::vector<int8_t> coarsen(const CsrHost &A, const std::vector<uint8_t> &S, int type,
+                            int second_pass, uint64_t seed) {
+    if (type == COARSEN_PMIS)
+        return pmis_coarsen(A, S, seed);
+    return rs_coarsen(A, S, second_pass);
+}
+
+CsrHost interpolate(const CsrHost &A, const std::vector<uint8_t> &S,
+                    const std::vector<int8_t> &cf, int type, float trunc, int pmax) {
+    int64_t nc = 0;
+    return interpolate_nc(A, S, cf, type, trunc, pmax, nc);
+}
 
 void estimate_eigs(const CsrHost &A, int iters, uint64_t seed, double &mn, double &mx) {
     const int64_t n = A.nrows;
@@ -700,24 +949,28 @@ void build(const CsrHost &A0, const SetupParams &p, Hierarchy &H) {
     H.A.push_back(A);
     while ((int)H.A.size() < p.max_levels && H.A.back().nrows > p.coarse_size) {
         const CsrHost &Af = H.A.back();
-        std::vector<uint8_t> S = strength(Af, p.strength, p.max_row_sum);
-        std::vector<int8_t> cf = rs_coarsen(Af, S, p.second_pass);
+        const int lev = (int)H.A.size() - 1;
+        const LevelKeys k = p.keys(lev);
+        std::vector<uint8_t> S = strength(Af, k.strength, k.max_row_sum);
+        const uint64_t seed = p.pmis_seed + 0x9E3779B97F4A7C15ull * (uint64_t)lev;
+        std::vector<int8_t> cf = coarsen(Af, S, k.coarsen_type, p.second_pass, seed);
         int64_t nc = 0;
         CsrHost P;
-        const int lev = (int)H.A.size() - 1;
         if (lev < p.agg_num_levels) {
-            // aggressive coarsening: a second Ruge-Stueben pass on the first-stage C
+            // aggressive coarsening: a second pass of the same scheme on the first-stage C
             // points with the long-range strength graph, then multipass interpolation
             std::vector<int64_t> cidx;
-            CsrHost G = second_strength(Af, S, cf, p.num_paths, cidx);
+            CsrHost G = second_strength(Af, S, cf, k.num_paths, cidx);
             std::vector<uint8_t> S2(G.col.size(), 1);
-            std::vector<int8_t> cf2 = rs_coarsen(G, S2, 0, true);
+            std::vector<int8_t> cf2 = k.coarsen_type == COARSEN_PMIS
+                                          ? pmis_coarsen(G, S2, seed ^ 0x5851F42D4C957F2Dull, true)
+                                          : rs_coarsen(G, S2, 0, true);
             for (int64_t i = 0; i < Af.nrows; ++i)
                 if (cf[i] == CPT && cf2[cidx[i]] != CPT)
                     cf[i] = FPT;
-            P = multipass(Af, S, cf, p.agg_trunc_factor, p.agg_pmax, nc);
+            P = multipass(Af, S, cf, k.agg_trunc_factor, k.agg_pmax, nc);
         } else {
-            P = interpolation(Af, S, cf, p, nc);
+            P = interpolate_nc(Af, S, cf, k.interp_type, k.trunc_factor, k.pmax, nc);
         }
         if (nc == 0 || nc >= Af.nrows)
             break;

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -22,6 +23,7 @@
 
 using xamg::amg::CsrHost;
 using xamg::amg::Hierarchy;
+using xamg::amg::LevelKeys;
 using xamg::amg::SetupParams;
 
 namespace {
@@ -73,6 +75,11 @@ void xamg_amg_default_params(xamg_amg_params *p) {
     p->agg_pmax = d.agg_pmax;
     p->eig_iters = d.eig_iters;
     p->eig_seed = d.eig_seed;
+    p->coarsen_type = d.coarsen_type;
+    p->interp_type = d.interp_type;
+    p->agg_interp_type = d.agg_interp_type;
+    p->nlev_keys = 0;
+    p->lev_keys = nullptr;
 }
 
 int xamg_amg_setup(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
@@ -102,6 +109,33 @@ int xamg_amg_setup(int64_t n, const uint32_t *row, const uint32_t *col, const do
         sp.agg_pmax = p->agg_pmax;
         sp.eig_iters = p->eig_iters;
         sp.eig_seed = p->eig_seed;
+        sp.coarsen_type = p->coarsen_type;
+        sp.interp_type = p->interp_type;
+        sp.agg_interp_type = p->agg_interp_type;
+        if (p->nlev_keys < 0 || (p->nlev_keys > 0 && !p->lev_keys))
+            return api_fail(-1, "xamg_amg_setup: bad per-level keys");
+        for (int l = 0; l < p->nlev_keys; ++l) {
+            const xamg_amg_level_keys &q = p->lev_keys[l];
+            LevelKeys k;
+            k.strength = q.strength;
+            k.max_row_sum = q.max_row_sum;
+            k.trunc_factor = q.trunc_factor;
+            k.pmax = q.pmax;
+            k.coarsen_type = q.coarsen_type;
+            k.interp_type = q.interp_type;
+            k.agg_interp_type = q.agg_interp_type;
+            k.agg_trunc_factor = q.agg_trunc_factor;
+            k.agg_pmax = q.agg_pmax;
+            k.num_paths = std::max(1, (int)q.num_paths);
+            sp.per_level.push_back(k);
+        }
+    }
+    for (int l = -1; l < (int)sp.per_level.size(); ++l) {
+        const LevelKeys k = sp.keys(l);
+        if (!xamg::amg::coarsen_supported(k.coarsen_type) ||
+            !xamg::amg::interp_supported(k.interp_type) ||
+            !xamg::amg::agg_interp_supported(k.agg_interp_type))
+            return api_fail(-3, "xamg_amg_setup: unsupported coarsening or interpolation type");
     }
     auto *h = new xamg_amg_s();
     xamg::amg::build(A, sp, h->H);
@@ -180,6 +214,69 @@ int xamg_amg_to_device(xamg_amg H, xh_hier *out) {
     return 0;
 }
 
+namespace {
+int host_csr(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+             CsrHost &A) {
+    if (n <= 0 || !row || !col || !val)
+        return -1;
+    A.nrows = A.ncols = n;
+    A.row.assign(row, row + n + 1);
+    A.col.assign(col, col + row[n]);
+    A.val.assign(val, val + row[n]);
+    for (uint32_t c : A.col)
+        if ((int64_t)c >= n)
+            return -1;
+    return 0;
+}
+} // namespace
+
+int xamg_amg_strength(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                      float theta, float max_row_sum, uint8_t *strong) {
+    CsrHost A;
+    if (host_csr(n, row, col, val, A) || !strong)
+        return api_fail(-1, "xamg_amg_strength: bad arguments");
+    std::vector<uint8_t> S = xamg::amg::strength_of(A, theta, max_row_sum);
+    std::copy(S.begin(), S.end(), strong);
+    return 0;
+}
+
+int xamg_amg_split(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                   const uint8_t *strong, int coarsen_type, uint64_t seed, int8_t *cf) {
+    CsrHost A;
+    if (host_csr(n, row, col, val, A) || !strong || !cf)
+        return api_fail(-1, "xamg_amg_split: bad arguments");
+    if (!xamg::amg::coarsen_supported(coarsen_type))
+        return api_fail(-3, "xamg_amg_split: unsupported coarsening type");
+    std::vector<uint8_t> S(strong, strong + A.nnz());
+    std::vector<int8_t> c = xamg::amg::coarsen(A, S, coarsen_type, 0, seed);
+    std::copy(c.begin(), c.end(), cf);
+    return 0;
+}
+
+int xamg_amg_interp(int64_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                    const uint8_t *strong, const int8_t *cf, int interp_type, float trunc,
+                    int pmax, xamg_amg *out) {
+    CsrHost A;
+    if (host_csr(n, row, col, val, A) || !strong || !cf || !out)
+        return api_fail(-1, "xamg_amg_interp: bad arguments");
+    if (!xamg::amg::interp_supported(interp_type) && interp_type != 4)
+        return api_fail(-3, "xamg_amg_interp: unsupported interpolation type");
+    std::vector<uint8_t> S(strong, strong + A.nnz());
+    std::vector<int8_t> c(cf, cf + n);
+    auto *h = new xamg_amg_s();
+    h->H.A.push_back(A);
+    h->H.P.push_back(xamg::amg::interpolate(A, S, c, interp_type, trunc, pmax));
+    h->H.R.push_back(xamg::amg::transpose(h->H.P[0]));
+    CsrHost Ac;
+    Ac.nrows = Ac.ncols = h->H.P[0].ncols;
+    Ac.row.assign(Ac.nrows + 1, 0);
+    h->H.A.push_back(Ac);
+    h->H.min_eig.assign(2, 0.0);
+    h->H.max_eig.assign(2, 0.0);
+    *out = h;
+    return 0;
+}
+
 int xamg_amg_free(xamg_amg H) {
     delete H;
     return 0;
@@ -252,11 +349,72 @@ namespace {
 
 using ParamList = std::map<std::string, std::string>;
 
+// one level-ranged override "key=value@levX[-Y]" (examples/common/cmdline/cmdline.h:79-97):
+// levels lo..hi, hi = INT32_MAX for "E" (the last level)
+struct LevelOverride {
+    std::string key, value;
+    int lo = 0, hi = 0;
+};
+
 struct GlobalParams {
     std::map<std::string, ParamList> roles;
-    // per-role level overrides "value@levX-Y" (examples/common/cmdline/cmdline.h:79-97)
-    std::map<std::string, std::vector<std::pair<std::string, std::string>>> overrides;
+    std::map<std::string, std::vector<LevelOverride>> overrides;
 };
+
+// "value@levX", "value@levX-Y" or "value@levX-E" -> value and the level range
+bool parse_level_value(const std::string &v, LevelOverride &o) {
+    const size_t at = v.rfind("@lev");
+    if (at == std::string::npos || at + 4 >= v.size())
+        return false;
+    o.value = v.substr(0, at);
+    const std::string r = v.substr(at + 4);
+    const size_t dash = r.find('-');
+    auto num = [](const std::string &t, int &out) {
+        if (t.empty() || t.find_first_not_of("0123456789") != std::string::npos)
+            return false;
+        out = std::atoi(t.c_str());
+        return true;
+    };
+    if (!num(r.substr(0, dash), o.lo))
+        return false;
+    if (dash == std::string::npos) {
+        o.hi = o.lo;
+    } else {
+        const std::string y = r.substr(dash + 1);
+        if (y == "E")
+            o.hi = INT32_MAX;
+        else if (!num(y, o.hi))
+            return false;
+    }
+    return !o.value.empty() && o.hi >= o.lo;
+}
+
+// Keys a level override may set, per role.  Smoother roles: the keys of the smoother
+// families (xamg_params_details.h:146-236) the GPU smoothers read.  Preconditioner: the
+// MultiGrid keys the host setup honours level by level (hypre_per_level_hierarchy,
+// src/hypre/detail/hypre_hierarchy.inl:54-108).  Every other key or role is rejected.
+bool override_key_ok(const std::string &role, const std::string &key) {
+    static const char *smoother[] = {"method", "max_iters", "relax_factor", "polynomial_order",
+                                     "spectrum_fraction", "convergence_check",
+                                     "convergence_info", "abs_tolerance", "rel_tolerance"};
+    static const char *mg[] = {"mg_coarsening_type", "mg_interpolation_type",
+                               "mg_agg_interpolation_type", "mg_strength_threshold",
+                               "mg_max_row_sum", "mg_trunc_factor", "mg_Pmax_elements",
+                               "mg_agg_trunc_factor", "mg_agg_Pmax_elements", "mg_num_paths",
+                               "mg_nonGalerkin_tol"};
+    if (role == "pre_smoother" || role == "post_smoother") {
+        for (const char *k : smoother)
+            if (key == k)
+                return true;
+        return false;
+    }
+    if (role == "preconditioner") {
+        for (const char *k : mg)
+            if (key == k)
+                return true;
+    }
+    return false;
+}
 
 std::string fmt_float(float v) {
     char b[64];
@@ -435,6 +593,23 @@ bool nv_supported(int nv) {
     return nv == 1 || nv == 2 || nv == 4 || nv == 8 || nv == 16 || nv == 32 || nv == 64;
 }
 
+// coarsening / interpolation types and Galerkin products the host setup implements
+int check_setup_keys(const LevelKeys &k, float non_galerkin_tol, const std::string &where) {
+    if (!xamg::amg::coarsen_supported(k.coarsen_type))
+        return api_fail(-3, "mg_coarsening_type " + std::to_string(k.coarsen_type) + where +
+                                " is not supported (6 Falgout/RS, 8 PMIS)");
+    if (!xamg::amg::interp_supported(k.interp_type))
+        return api_fail(-3, "mg_interpolation_type " + std::to_string(k.interp_type) + where +
+                                " is not supported (0 classical, 6 extended+i)");
+    if (!xamg::amg::agg_interp_supported(k.agg_interp_type))
+        return api_fail(-3, "mg_agg_interpolation_type " + std::to_string(k.agg_interp_type) +
+                                where + " is not supported (4 multipass)");
+    if (non_galerkin_tol != 0.f)
+        return api_fail(-3, "mg_nonGalerkin_tol" + where +
+                                " must be 0 (Galerkin coarse operators only)");
+    return 0;
+}
+
 int build_xh_params(GlobalParams &g, xh_params &p, SetupParams &sp, bool &mg) {
     std::memset(&p, 0, sizeof p);
     if (!g.roles.count("solver"))
@@ -473,6 +648,43 @@ int build_xh_params(GlobalParams &g, xh_params &p, SetupParams &sp, bool &mg) {
         sp.num_paths = std::max(1, (int)geti(l, "mg_num_paths", 3));
         sp.agg_trunc_factor = getf(l, "mg_agg_trunc_factor", 0.25f);
         sp.agg_pmax = (int)geti(l, "mg_agg_Pmax_elements", 4);
+        sp.coarsen_type = (int)geti(l, "mg_coarsening_type", 6);
+        sp.interp_type = (int)geti(l, "mg_interpolation_type", 0);
+        sp.agg_interp_type = (int)geti(l, "mg_agg_interpolation_type", 4);
+        if ((rc = check_setup_keys(sp.keys(-1), getf(l, "mg_nonGalerkin_tol", 0.f), "")))
+            return rc;
+        // level overrides of the setup keys take effect with a per-level hierarchy, as in
+        // the reference, where get_full_hierarchy reads the list without a level
+        // (hypre_hierarchy.inl:31-52) and get_per_level_hierarchy the list of each level
+        // (:54-108)
+        auto ov = g.overrides.find("preconditioner");
+        if (geti(l, "hypre_per_level_hierarchy", 0) && ov != g.overrides.end() &&
+            !ov->second.empty()) {
+            int nlev = 0;
+            for (auto &o : ov->second)
+                nlev = std::max(nlev, std::min(o.hi, sp.max_levels - 1) + 1);
+            for (int lev = 0; lev < nlev; ++lev) {
+                ParamList pl = l;
+                for (auto &o : ov->second)
+                    if (o.lo <= lev && lev <= o.hi)
+                        pl[o.key] = o.value;
+                LevelKeys k;
+                k.strength = getf(pl, "mg_strength_threshold", 0.5f);
+                k.max_row_sum = getf(pl, "mg_max_row_sum", 0.9f);
+                k.trunc_factor = getf(pl, "mg_trunc_factor", 0.25f);
+                k.pmax = (int)geti(pl, "mg_Pmax_elements", 4);
+                k.coarsen_type = (int)geti(pl, "mg_coarsening_type", 6);
+                k.interp_type = (int)geti(pl, "mg_interpolation_type", 0);
+                k.agg_interp_type = (int)geti(pl, "mg_agg_interpolation_type", 4);
+                k.agg_trunc_factor = getf(pl, "mg_agg_trunc_factor", 0.25f);
+                k.agg_pmax = (int)geti(pl, "mg_agg_Pmax_elements", 4);
+                k.num_paths = std::max(1, (int)geti(pl, "mg_num_paths", 3));
+                if ((rc = check_setup_keys(k, getf(pl, "mg_nonGalerkin_tol", 0.f),
+                                           " on level " + std::to_string(lev))))
+                    return rc;
+                sp.per_level.push_back(k);
+            }
+        }
     } else {
         p.graph = 1;
         p.hsgs_block_rows = 64;
@@ -482,6 +694,40 @@ int build_xh_params(GlobalParams &g, xh_params &p, SetupParams &sp, bool &mg) {
                    std::make_pair("coarse_grid_solver", &p.coarse)})
         if (g.roles.count(q.first) && (rc = role_of(g.roles[q.first], *q.second)))
             return rc;
+    // per-level smoother roles: params.get("pre_smoother", l) (mg.inl:113-131) is the
+    // role's list with the overrides whose range holds l applied in the order given, and
+    // the defaults of the resulting family filled in
+    int nlev = 0;
+    for (const char *r : {"pre_smoother", "post_smoother"}) {
+        auto ov = g.overrides.find(r);
+        if (ov != g.overrides.end())
+            for (auto &o : ov->second)
+                nlev = std::max(nlev, std::min(o.hi, XH_MAX_LEVELS - 1) + 1);
+    }
+    for (int lev = 0; lev < nlev; ++lev) {
+        for (auto q : {std::make_pair("pre_smoother", &p.pre_lev[lev]),
+                       std::make_pair("post_smoother", &p.post_lev[lev])}) {
+            ParamList pl = g.roles.count(q.first) ? g.roles[q.first] : ParamList();
+            auto ov = g.overrides.find(q.first);
+            if (ov != g.overrides.end())
+                for (auto &o : ov->second)
+                    if (o.lo <= lev && lev <= o.hi)
+                        pl[o.key] = o.value;
+            family_defaults(pl, q.first);
+            if ((rc = role_of(pl, *q.second)))
+                return rc;
+        }
+    }
+    p.nlev_roles = nlev;
+    // the GPU smoothers run without a convergence check (the reference default for
+    // smoother lists, xamg_params_details.h:164-170)
+    auto checked = [](const xh_role &r) { return r.method != XH_NONE && r.conv_check != 0; };
+    bool bad = checked(p.pre) || checked(p.post) || checked(p.coarse);
+    for (int lev = 0; lev < nlev; ++lev)
+        bad = bad || checked(p.pre_lev[lev]) || checked(p.post_lev[lev]);
+    if (mg && bad)
+        return api_fail(-3, "convergence_check on a smoother or coarse-grid role is not "
+                            "supported (smoothers run a fixed number of sweeps)");
     return 0;
 }
 
@@ -589,11 +835,24 @@ int XAMG_param_override_apply(char *role, XAMG_param_id pid, XAMG_param_override
     GlobalParams *g = lookup(g_params, pid);
     if (!o || !g || !role)
         return api_fail(-1, "bad override/parameter handle");
-    // stored for reference; the classical setup of this build has one coarsening and
-    // interpolation scheme for all levels, so level-ranged overrides of hypre keys
-    // (e.g. mg_coarsening_type=8@lev0-1) do not change the hierarchy
-    for (auto &kv : *o)
-        g->overrides[role].push_back(kv);
+    // params::add_override (api/c/xamg_c_api.cpp:256-260): each value carries its level
+    // range; the whole set is checked before any of it is stored
+    std::vector<LevelOverride> parsed;
+    for (auto &kv : *o) {
+        LevelOverride lo;
+        lo.key = kv.first;
+        if (!parse_level_value(kv.second, lo))
+            return api_fail(-1, "override '" + kv.first + "=" + kv.second +
+                                    "': value must end in @levX, @levX-Y or @levX-E");
+        if (!override_key_ok(role, lo.key))
+            return api_fail(-3, std::string("override of '") + lo.key + "' on role '" + role +
+                                    "' is not supported (level overrides apply to the "
+                                    "pre_smoother / post_smoother keys and to the MultiGrid "
+                                    "setup keys of the preconditioner)");
+        parsed.push_back(lo);
+    }
+    auto &dst = g->overrides[role];
+    dst.insert(dst.end(), parsed.begin(), parsed.end());
     return 0;
 }
 
@@ -842,6 +1101,45 @@ static int solver_create(XAMG_matrix_id mid, XAMG_param_id pid, int nv, size_t x
     s->rel_res.assign(nv, 0.0);
     g_solvers.push_back(std::move(s));
     *sid = g_solvers.size() - 1;
+    return 0;
+}
+
+int xamg_param_resolve(size_t pid, xh_params *prm, xamg_amg_params *setup,
+                       xamg_amg_level_keys *lev_keys) {
+    GlobalParams *g = lookup(g_params, pid);
+    if (!g || !prm)
+        return api_fail(-1, "xamg_param_resolve: bad arguments");
+    SetupParams sp;
+    bool mg = false;
+    int rc = build_xh_params(*g, *prm, sp, mg);
+    if (rc)
+        return rc;
+    if (setup) {
+        setup->strength = sp.strength;
+        setup->max_row_sum = sp.max_row_sum;
+        setup->coarse_size = sp.coarse_size;
+        setup->max_levels = sp.max_levels;
+        setup->trunc_factor = sp.trunc_factor;
+        setup->pmax = sp.pmax;
+        setup->second_pass = sp.second_pass;
+        setup->agg_num_levels = sp.agg_num_levels;
+        setup->num_paths = sp.num_paths;
+        setup->agg_trunc_factor = sp.agg_trunc_factor;
+        setup->agg_pmax = sp.agg_pmax;
+        setup->eig_iters = sp.eig_iters;
+        setup->eig_seed = sp.eig_seed;
+        setup->coarsen_type = sp.coarsen_type;
+        setup->interp_type = sp.interp_type;
+        setup->agg_interp_type = sp.agg_interp_type;
+        setup->nlev_keys = std::min((int)sp.per_level.size(), XH_MAX_LEVELS);
+        setup->lev_keys = nullptr;
+        for (int l = 0; lev_keys && l < setup->nlev_keys; ++l) {
+            const LevelKeys &k = sp.per_level[l];
+            lev_keys[l] = {k.strength,     k.max_row_sum,     k.trunc_factor, k.pmax,
+                           k.coarsen_type, k.interp_type,     k.agg_interp_type,
+                           k.agg_trunc_factor, k.agg_pmax, k.num_paths};
+        }
+    }
     return 0;
 }
 

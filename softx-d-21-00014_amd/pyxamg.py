@@ -26,6 +26,9 @@ METHODS = {
 }
 
 
+MAX_LEVELS = 32  # XH_MAX_LEVELS
+
+
 class Role(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_iters", C.c_int32), ("abs_tol", C.c_float),
                 ("rel_tol", C.c_float), ("conv_check", C.c_int32), ("relax_factor", C.c_float),
@@ -37,7 +40,9 @@ class Params(C.Structure):
                 ("coarse", Role), ("mg_cycle", C.c_int32), ("mg_prec", C.c_int32),
                 ("hsgs_ncores", C.c_int32), ("hsgs_block_rows", C.c_int32),
                 ("coarse_fp64", C.c_int32), ("index_mode", C.c_int32), ("graph", C.c_int32),
-                ("true_res_period", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("true_res_period", C.c_int32), ("reserved", C.c_int32 * 3),
+                ("nlev_roles", C.c_int32), ("pre_lev", Role * MAX_LEVELS),
+                ("post_lev", Role * MAX_LEVELS)]
 
 
 class AmgParams(C.Structure):
@@ -45,7 +50,18 @@ class AmgParams(C.Structure):
                 ("max_levels", C.c_int32), ("trunc_factor", C.c_float), ("pmax", C.c_int32),
                 ("second_pass", C.c_int32), ("agg_num_levels", C.c_int32),
                 ("num_paths", C.c_int32), ("agg_trunc_factor", C.c_float),
-                ("agg_pmax", C.c_int32), ("eig_iters", C.c_int32), ("eig_seed", C.c_uint64)]
+                ("agg_pmax", C.c_int32), ("eig_iters", C.c_int32), ("eig_seed", C.c_uint64),
+                ("coarsen_type", C.c_int32), ("interp_type", C.c_int32),
+                ("agg_interp_type", C.c_int32), ("nlev_keys", C.c_int32),
+                ("lev_keys", C.c_void_p)]
+
+
+class AmgLevelKeys(C.Structure):
+    _fields_ = [("strength", C.c_float), ("max_row_sum", C.c_float),
+                ("trunc_factor", C.c_float), ("pmax", C.c_int32), ("coarsen_type", C.c_int32),
+                ("interp_type", C.c_int32), ("agg_interp_type", C.c_int32),
+                ("agg_trunc_factor", C.c_float), ("agg_pmax", C.c_int32),
+                ("num_paths", C.c_int32)]
 
 
 def role(method, max_iters=1, abs_tol=0.0, rel_tol=0.0, conv_check=0, relax_factor=0.99,
@@ -150,6 +166,9 @@ def _load():
         "xamg_amg_estimate_eigs": [vp, C.c_int, u64],
         "xamg_amg_to_device": [vp, P(vp)],
         "xamg_amg_free": [vp],
+        "xamg_amg_strength": [i64, vp, vp, vp, C.c_float, C.c_float, vp],
+        "xamg_amg_split": [i64, vp, vp, vp, vp, C.c_int, u64, vp],
+        "xamg_amg_interp": [i64, vp, vp, vp, vp, vp, C.c_int, C.c_float, C.c_int, P(vp)],
         "xamg_cube": [i64, i64, i64, vp, vp, vp],
         "xamg_lognormal_field": [u64, f64, i64, vp],
         "xamg_varcoef7": [i64, i64, i64, vp, vp, vp, vp],
@@ -378,13 +397,56 @@ def amg_default_params():
     return p
 
 
+def amg_strength(row, col, val, theta=0.5, max_row_sum=0.9):
+    """strength flags per nonzero of the host AMG setup (uint8, 1 strong)"""
+    s = np.zeros(len(col), np.uint8)
+    check(host().xamg_amg_strength(len(row) - 1, _ptr(row), _ptr(col), _ptr(val), theta,
+                                   max_row_sum, _ptr(s)), "xamg_amg_strength")
+    return s
+
+
+def amg_split(row, col, val, strong, coarsen_type=6, seed=2747):
+    """C/F splitting of the host AMG setup: int8, 1 C, -1 F"""
+    cf = np.zeros(len(row) - 1, np.int8)
+    check(host().xamg_amg_split(len(row) - 1, _ptr(row), _ptr(col), _ptr(val), _ptr(strong),
+                                coarsen_type, seed, _ptr(cf)), "xamg_amg_split")
+    return cf
+
+
+def amg_interp(row, col, val, strong, cf, interp_type=0, trunc=0.0, pmax=0):
+    """interpolation P of a splitting: (nrows, ncols, row, col, val)"""
+    h = C.c_void_p()
+    cf = np.ascontiguousarray(cf, np.int8)
+    check(host().xamg_amg_interp(len(row) - 1, _ptr(row), _ptr(col), _ptr(val), _ptr(strong),
+                                 _ptr(cf), interp_type, trunc, pmax, C.byref(h)),
+          "xamg_amg_interp")
+    a = Amg.__new__(Amg)
+    a.h = h
+    a._keep = None
+    return a.level(0, 1)
+
+
 class Amg:
     """Host-side classical AMG hierarchy (libxamg.so)."""
 
-    def __init__(self, row, col, val, params=None, **kw):
+    def __init__(self, row, col, val, params=None, level_keys=None, **kw):
+        """level_keys: optional list of dicts, one per level from level 0, each with any
+        of the AmgLevelKeys fields (the others take the list-wide values)"""
         p = params or amg_default_params()
         for k, v in kw.items():
             setattr(p, k, v)
+        self._lev = None
+        if level_keys:
+            arr = (AmgLevelKeys * len(level_keys))()
+            for l, d in enumerate(level_keys):
+                q = arr[l]
+                for f in ("strength", "max_row_sum", "trunc_factor", "pmax", "coarsen_type",
+                          "interp_type", "agg_interp_type", "agg_trunc_factor", "agg_pmax",
+                          "num_paths"):
+                    setattr(q, f, d.get(f, getattr(p, f)))
+            self._lev = arr
+            p.nlev_keys = len(level_keys)
+            p.lev_keys = C.cast(arr, C.c_void_p)
         self._keep = (row, col, val)
         self.h = C.c_void_p()
         n = len(row) - 1
@@ -784,4 +846,23 @@ def default_params(method="MergedPBiCGStab", precond="MultiGrid", smoother="Cheb
     p.index_mode = index_mode
     p.graph = graph
     p.true_res_period = true_res_period
+    p.nlev_roles = 0
+    return p
+
+
+def set_level_roles(p, pre=None, post=None):
+    """Per-level smoother roles: pre[l] / post[l] (Role objects or None for the list-wide
+    role) become the roles of level l, the other levels keep p.pre / p.post."""
+    pre, post = list(pre or []), list(post or [])
+    n = max(len(pre), len(post))
+    if n > MAX_LEVELS:
+        raise ValueError("at most %d levels" % MAX_LEVELS)
+    for l in range(n):
+        a = pre[l] if l < len(pre) and pre[l] is not None else p.pre
+        b = post[l] if l < len(post) and post[l] is not None else p.post
+        p.pre_lev[l] = Role(a.method, a.max_iters, a.abs_tol, a.rel_tol, a.conv_check,
+                            a.relax_factor, a.poly_order, a.spectrum_fraction)
+        p.post_lev[l] = Role(b.method, b.max_iters, b.abs_tol, b.rel_tol, b.conv_check,
+                             b.relax_factor, b.poly_order, b.spectrum_fraction)
+    p.nlev_roles = n
     return p
