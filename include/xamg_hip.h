@@ -72,6 +72,10 @@ int xh_synchronize(void);
 /* dst[0:count] = src[0:count] (fp64) with the library's copy kernel, then synchronize;
  * a kernel of known traffic (8*count read + 8*count written) for counter calibration */
 int xh_copy_device(double *dst, const double *src, size_t count);
+/* the same copy launched `reps` times on the library stream, timed with HIP events on that
+ * stream: average milliseconds per launch (the measured streaming ceiling of the device) */
+int xh_copy_device_timed(double *dst, const double *src, size_t count, int reps,
+                         double *ms_per_launch);
 /* dptr[0:bytes] = 0 on the library stream, then synchronize (XAMG_vector_set_val_d(0)) */
 int xh_fill_zero(void *dptr, size_t bytes);
 
@@ -120,6 +124,16 @@ int xh_matrix_fold(xh_matrix A, int64_t set, int64_t *fold);
  * number of coded entries.  xh_grid_schedule: the work items of one launch, 4 values
  * each (first line y, first line z, x begin, x end). */
 int xh_matrix_grid(xh_matrix A, int32_t *dims);
+/* The 7-point stencil layout (no reference counterpart): a matrix with a grid view whose
+ * every row holds exactly the diagonal and all its in-grid neighbours, in the order -z,
+ * -y, -x, +x, +y, +z (the reference generators' order), also stores its values as 8
+ * slots per row {C, -z, -y, -x, +x, +y, +z, unused}; with "spmv_st7_enable" its SpMVs for
+ * nv >= 8 sweep GY x GZ grid lines per wave along x, reusing x(c-1), x(c) from registers
+ * and the rows of the wave's own lines through cross-lane moves (same products, same
+ * order: identical results).  Knobs "spmv_st7_waves" (waves per launch, default 3072),
+ * "spmv_st7_min_seg" (default 32).
+ * xh_matrix_layouts: flags bit 0 grid view, bit 1 stencil layout, bit 2 u16 columns. */
+int xh_matrix_layouts(xh_matrix A, int32_t *flags);
 int xh_csr_grid(int64_t nrows, const uint32_t *row, const uint32_t *col, int32_t *dims,
                 uint8_t *code, int64_t *coded);
 int xh_grid_schedule(int32_t nx, int32_t ny, int32_t nz, int prec, int nv, int dots,
@@ -250,6 +264,30 @@ int xh_solver_stats(xh_solver S, int32_t *iters, int32_t *converged, double *abs
 int xh_solver_history(xh_solver S, double *hist, int32_t max_checks, int32_t *nchecks);
 /* one preconditioner application x = M^{-1} b on device vectors (x starts at zero) */
 int xh_solver_precond_apply(xh_solver S, const double *db, double *dx);
+/* x <- the finest level's pre-smoother applied `sweeps` times to A x = b from the given x
+ * (MultiGrid preconditioner with an fp64 finest level; Chebyshev, Jacobi or HSGS role):
+ * the pre-smoothing step of MultiGrid::MG_cycle (mg.inl:197-199) on its own.  No reference
+ * counterpart as an entry point; used to damp the high-frequency error of a learned
+ * initial guess before the solve. */
+int xh_solver_smooth(xh_solver S, const double *db, double *dx, int32_t sweeps);
+/* Per-kernel accounting of a solve: the reference's perf counters (src/io/perf.h:34-104
+ * count the bytes each operation reads and writes per solve) restated per GPU launch.
+ * Runs the init step and exactly `iters` iterations from x = dx (tolerances ignored, no
+ * hipGraph: every launch is enqueued on its own, bracketed by two HIP events on the
+ * solver's stream) and sums, per (kernel, level), the launches, their algorithmic HBM
+ * bytes (each operand read once, each result written once) and their event-timed
+ * milliseconds.  level: hierarchy level of a V-cycle launch, -1 for the outer Krylov
+ * method.  Entries are sorted by time; *nstats receives their total count (only the first
+ * max_stats are written).  The solver's statistics afterwards are those of this run. */
+typedef struct {
+    char kernel[80];
+    int32_t level;
+    int32_t launches;
+    double bytes;
+    double ms;
+} xh_kernel_stat;
+int xh_solver_profile(xh_solver S, const double *db, double *dx, int32_t iters,
+                      xh_kernel_stat *stats, int32_t max_stats, int32_t *nstats);
 /* renew the solver-role parameters (tolerances, max_iters, convergence_check) of an
  * existing solver: base_solver::renew_params (src/solvers/solver.h:277-288); the method
  * itself cannot change */

@@ -96,7 +96,21 @@ def lib():
         _lib.xo_solver_solve.restype = C.c_int
         _lib.xo_solver_free.argtypes = [vp]
         _lib.xo_solver_free.restype = None
+        _lib.xo_set_threads.argtypes = [C.c_int]
+        _lib.xo_set_threads.restype = C.c_int
+        _lib.xo_get_threads.argtypes = []
+        _lib.xo_get_threads.restype = C.c_int
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops"""
+    if lib().xo_set_threads(int(n)):
+        raise ValueError("thread count must be >= 1")
+
+
+def threads():
+    return lib().xo_get_threads()
 
 
 def _p(a):

@@ -30,6 +30,7 @@
 #include <memory>
 #include <type_traits>
 #include <vector>
+#include <omp.h>
 
 namespace xo {
 
@@ -1908,6 +1909,15 @@ static int run_solve(const xo_hier *H, const xo_params *prm, int nv, const doubl
 using namespace xo;
 
 extern "C" {
+
+/* OpenMP threads of the oracle's parallel loops (the reference's cores per rank) */
+int xo_set_threads(int n) {
+    if (n < 1)
+        return -1;
+    omp_set_num_threads(n);
+    return 0;
+}
+int xo_get_threads(void) { return omp_get_max_threads(); }
 
 void xo_spmv_d(const xo_csr *A, int nv, const double *x, double *y, int accumulate) {
     Csr<double> m = make_csr<double>(A);

@@ -150,6 +150,11 @@ int xo_solver_solve(xo_solver S, int nv, const double *b, double *x, int32_t *it
                     int32_t *converged, double *abs_res, double *rel_res);
 void xo_solver_free(xo_solver S);
 
+/* OpenMP threads of the oracle's parallel loops; results do not depend on them (the dot
+ * partition is dot_ncores, not the thread count) */
+int xo_set_threads(int n);
+int xo_get_threads(void);
+
 /* one application of the MultiGrid preconditioner to b (x = M^{-1} b, x starts at 0) */
 int xo_mg_apply(const xo_hier *H, const xo_params *prm, int nv, const double *b, double *x);
 

@@ -44,6 +44,10 @@ struct Mat {
     // for a column that is not the grid neighbour row + a*gx*gy + b*gx + d
     int32_t gx = 0, gy = 0, gz = 0;
     uint8_t *gcode = nullptr;
+    // complete 7-point stencil on the grid view (every row: the diagonal and all in-grid
+    // neighbours in the order -z, -y, -x, +x, +y, +z): the values again as st7[i][8],
+    // slots {C, -z, -y, -x, +x, +y, +z, unused}, 0 where a neighbour is outside the grid
+    void *st7 = nullptr;
     // core-block matrix of hybrid GS (entries inside each row's block of core_B rows only):
     // core_B > 0, and core_emax = the largest entry count of one block
     int64_t core_B = 0, core_emax = 0;
@@ -122,6 +126,19 @@ __host__ __device__ inline void grid_item(const GridSched &g, int k, int &yb, in
     c1 = c0 + g.seglen < g.nx ? c0 + g.seglen : g.nx;
 }
 
+// Work items of the 7-point stencil SpMV: blocks of GY x GZ grid lines (one wave each),
+// each line cut into nseg segments of seglen rows; item k = (seg, zb, yb), yb fastest.
+// Wave w takes items w, w + nwaves, w + 2 nwaves, ...
+struct StSched {
+    int32_t nx = 0, ny = 0, nz = 0, nyb = 0, nzb = 0, seglen = 0, nseg = 1;
+    int64_t nitems = 0;
+};
+struct StLaunch {
+    int grid = 0;
+    StSched g;
+};
+StLaunch st7_plan(int nx, int ny, int nz, int gy, int gz, int cap);
+
 // Tile schedule and grid of one SpMV launch over n rows with nnz nonzeros and fold F
 // (0: none): rpb rows per workgroup pass, at most cap workgroups, row_bytes bytes of x
 // per row.
@@ -145,8 +162,39 @@ struct Tuning {
     int grid_enable = 0;
     int grid_items = 2048;          // target work items (tile x segment) per launch
     int grid_min_seg = 32;          // shortest x segment of a work item
+    // 1: matrices with a complete 7-point stencil view run the register-window line
+    // sweep (k_spmv_st7) for nv >= 8
+    int st7_enable = 0;
+    int st7_waves = 3072;           // waves of one launch: 3 per SIMD of 256 CUs
+    int st7_min_seg = 32;           // shortest x segment of a work item
 };
 extern Tuning g_tune;
+
+// ---- launch accounting (xh_solver_profile) ----
+// While a profile solve runs, every launch helper of the solve path brackets its kernel
+// with two HIP events on its stream and records the kernel, the hierarchy level the
+// runtime is at (-1: the outer Krylov method) and the launch's algorithmic HBM bytes:
+// each operand the launch needs read once and each result written once (matrix values,
+// column indices, row pointers and panel bases; vectors; partial sums).
+bool ledger_active();
+void ledger_open(hipStream_t s);
+void ledger_close(hipStream_t s, const std::string &kernel, double bytes);
+struct LedgerScope {
+    hipStream_t s;
+    bool on;
+    std::string kernel;
+    double bytes = 0.0;
+    explicit LedgerScope(hipStream_t st) : s(st), on(ledger_active()) {
+        if (on)
+            ledger_open(s);
+    }
+    ~LedgerScope() {
+        if (on)
+            ledger_close(s, kernel, bytes);
+    }
+};
+const char *prec_name(int prec);
+const char *idx_name(int idx);
 namespace unet {
 void set_fuse_up(int on); // 0: U-Net inference materialises the upsampled tensors
 void set_conv_tile(int on); // 0: no LDS-staged convolution tiles
@@ -346,6 +394,8 @@ void convert(int64_t count, int from_prec, const void *src, void *dst, const int
              hipStream_t s);
 void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *done,
               hipStream_t s);
+// x = u + x
+void add_vec(int64_t count, int prec, const void *u, void *x, const int *done, hipStream_t s);
 void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s);
 
 } // namespace xh

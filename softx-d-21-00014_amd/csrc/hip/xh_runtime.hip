@@ -55,11 +55,69 @@ int ensure_init() {
     return xh_init(0);
 }
 
+// Launch records of a profile solve: kernel, level, iteration, algorithmic bytes and the
+// two events around the launch.  Events come from a pool that lives as long as the ledger.
+struct LedgerRec {
+    std::string kernel;
+    int level, iter;
+    double bytes;
+    hipEvent_t e0, e1;
+};
+struct Ledger {
+    std::vector<LedgerRec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t next = 0;
+    int level = -1, iter = 0;
+    hipEvent_t open = nullptr;
+    bool failed = false;
+    hipEvent_t take() {
+        if (next == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) {
+                failed = true;
+                return nullptr;
+            }
+            pool.push_back(e);
+        }
+        return pool[next++];
+    }
+    ~Ledger() {
+        for (hipEvent_t e : pool)
+            (void)hipEventDestroy(e);
+    }
+};
+Ledger *g_ledger = nullptr;
+
+void ledger_level(int l) {
+    if (g_ledger)
+        g_ledger->level = l;
+}
+
 } // namespace
 
 // shared with the other sources of the library (xh_internal.hpp)
 int report_error(int code, const std::string &msg) { return fail(code, msg); }
 int runtime_init() { return ensure_init(); }
+
+bool ledger_active() { return g_ledger != nullptr; }
+
+void ledger_open(hipStream_t s) {
+    hipEvent_t e = g_ledger->take();
+    if (e && hipEventRecord(e, s) != hipSuccess)
+        g_ledger->failed = true;
+    g_ledger->open = e;
+}
+
+void ledger_close(hipStream_t s, const std::string &kernel, double bytes) {
+    Ledger &L = *g_ledger;
+    hipEvent_t e = L.take();
+    if (!e || !L.open || hipEventRecord(e, s) != hipSuccess) {
+        L.failed = true;
+        return;
+    }
+    L.recs.push_back({kernel, L.level, L.iter, bytes, L.open, e});
+    L.open = nullptr;
+}
 
 namespace {
 
@@ -200,6 +258,37 @@ int dmalloc(T **p, size_t count, size_t &acc) {
     return 0;
 }
 
+// The st7 value layout of a grid matrix (gx x gy x gz, entry codes `code`) when every row
+// holds exactly the diagonal and its in-grid neighbours in the order -z, -y, -x, +x, +y,
+// +z: v8[i*8 + s], slot s of {C, -z, -y, -x, +x, +y, +z, unused}, 0 for a neighbour
+// outside the grid.  Returns false for any other matrix.
+bool stencil7_values(const HostCsr &h, int32_t gx, int32_t gy, int32_t gz, const uint8_t *code,
+                     std::vector<double> &v8) {
+    // entry code 9(dz+1) + 3(dy+1) + (dx+1) of each slot
+    static const uint8_t slot_code[7] = {13, 4, 10, 12, 14, 16, 22};
+    const int64_t n = h.nrows;
+    if ((int64_t)gx * gy * gz != n)
+        return false;
+    v8.assign((size_t)n * 8, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t cx = i % gx, cy = (i / gx) % gy, cz = i / ((int64_t)gx * gy);
+        const bool present[7] = {true,        cz > 0,      cy > 0,     cx > 0,
+                                 cx + 1 < gx, cy + 1 < gy, cz + 1 < gz};
+        uint32_t j = h.row[i];
+        for (int s2 = 0; s2 < 7; ++s2) {
+            if (!present[s2])
+                continue;
+            if (j >= h.row[i + 1] || code[j] != slot_code[s2])
+                return false;
+            v8[(size_t)i * 8 + s2] = h.val[j];
+            ++j;
+        }
+        if (j != h.row[i + 1])
+            return false;
+    }
+    return true;
+}
+
 int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
     M = Mat();
     M.nrows = h.nrows;
@@ -305,6 +394,25 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
             M.gx = gx;
             M.gy = gy;
             M.gz = gz;
+            std::vector<double> v8;
+            if (stencil7_values(h, gx, gy, gz, code.data(), v8)) {
+                if (prec == F64) {
+                    double *d = nullptr;
+                    if ((rc = dmalloc(&d, v8.size(), acc)))
+                        return rc;
+                    XH_HIP(hipMemcpy(d, v8.data(), v8.size() * 8, hipMemcpyHostToDevice));
+                    M.st7 = d;
+                } else {
+                    std::vector<float> f8(v8.size());
+                    for (size_t q = 0; q < v8.size(); ++q)
+                        f8[q] = (float)v8[q];
+                    float *d = nullptr;
+                    if ((rc = dmalloc(&d, f8.size(), acc)))
+                        return rc;
+                    XH_HIP(hipMemcpy(d, f8.data(), f8.size() * 4, hipMemcpyHostToDevice));
+                    M.st7 = d;
+                }
+            }
         }
     }
     M.y_rows = h.nrows;
@@ -372,7 +480,8 @@ int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
 
 void free_mat(Mat &M) {
     for (void *p : {(void *)M.row, M.col, (void *)M.panel_base, M.val, M.inv_diag,
-                    M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind, (void *)M.gcode})
+                    M.inv_sqrt_diag, (void *)M.row_ind, (void *)M.col_ind, (void *)M.gcode,
+                    M.st7})
         if (p)
             (void)hipFree(p);
     M = Mat();
@@ -574,15 +683,19 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
             cheb_init_zero(L.n, nv, L.prec, b, L.A.inv_sqrt_diag, cheb[k - 1], L.sr,
                            k == 1 ? x : uc, k == 1, S->dflag, st);
         } else {
+            // r = isd*(b - A x), u = (c*r)*isd; for order 1 then x = u + x in a launch of
+            // its own: x is this SpMV's operand, which other rows still read, so it cannot
+            // be updated in place by the same launch
             SpmvArgs a = args(S);
             a.b = b;
             a.isd = L.A.inv_sqrt_diag;
             a.out2 = L.sr;
-            a.xupd = x;
             a.c = cheb[k - 1];
-            a.last = k == 1;
+            a.last = 0;
             a.x_zero = 0;
             spmv(L.A, nv, x, uc, OP_CHEB0, a, st);
+            if (k == 1)
+                add_vec(L.n * nv, L.prec, uc, x, S->dflag, st);
         }
         for (int i = k - 2; i >= 0; --i) {
             SpmvArgs a = args(S);
@@ -654,6 +767,7 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
     const int nv = S->nv;
     hipStream_t st = S->stream;
     const int nl = (int)S->lv.size();
+    ledger_level(l);
     if (cyc != 2) {
         smooth(S, L.pre, L, L.cheb_pre, x, b, x_zero);
         x_zero = false;
@@ -672,6 +786,7 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
     }
     spmv(L.R, nv, rt, H.b, OP_SET, args(S), st);
     if (l + 1 == nl - 1) {
+        ledger_level(l + 1);
         smooth(S, S->prm.coarse, H, H.cheb_coarse, H.x, H.b, true);
     } else {
         mg_cycle(S, l + 1, H.x, H.b, true, cyc);
@@ -680,6 +795,7 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
         else if (cyc == 2)
             mg_cycle(S, l + 1, H.x, H.b, false, 0);
     }
+    ledger_level(l);
     if (L.tprec == L.prec) {
         spmv(L.P, nv, H.x, x, x_zero ? OP_SET : OP_ADD, args(S), st);
     } else {
@@ -691,7 +807,15 @@ void mg_cycle(xh_solver_s *S, int l, void *x, const void *b, bool x_zero, int cy
 }
 
 // preconditioner application x = M^{-1} b, x zero-flagged on entry
+void precond_body(xh_solver_s *S, const double *b, double *x);
+
 void precond(xh_solver_s *S, const double *b, double *x) {
+    ledger_level(0);
+    precond_body(S, b, x);
+    ledger_level(-1);
+}
+
+void precond_body(xh_solver_s *S, const double *b, double *x) {
     const xh_role &pr = S->prm.precond;
     const int nv = S->nv;
     hipStream_t st = S->stream;
@@ -986,6 +1110,10 @@ int solve_impl(xh_solver_s *S, const double *db, double *dx) {
     h.hist_len = S->hist_len;
     XH_HIP(hipMemcpyAsync(S->st, S->hst, sizeof(SolverState), hipMemcpyHostToDevice, S->stream));
     S->dflag = &S->st->done;
+    if (g_ledger) {
+        g_ledger->iter = 0;
+        g_ledger->level = -1;
+    }
     enqueue_init(S, db, dx);
     int rc = check_launch();
     if (rc)
@@ -1015,6 +1143,8 @@ int solve_impl(xh_solver_s *S, const double *db, double *dx) {
         if (use_graph) {
             XH_HIP(hipGraphLaunch(replace ? S->gexec_rep : S->gexec, S->stream));
         } else {
+            if (g_ledger)
+                g_ledger->iter = it + 1;
             enqueue_iteration(S, dx, replace);
             if ((rc = check_launch()))
                 return rc;
@@ -1531,6 +1661,83 @@ int xh_solver_precond_apply(xh_solver S, const double *db, double *dx) {
     return 0;
 }
 
+int xh_solver_smooth(xh_solver S, const double *db, double *dx, int32_t sweeps) {
+    if (!S || !db || !dx || sweeps < 1)
+        return fail(XH_EINVAL, "xh_solver_smooth: bad arguments");
+    if (S->prm.precond.method != XH_MULTIGRID || S->lv.empty())
+        return fail(XH_ENOTSUP, "xh_solver_smooth: needs a MultiGrid preconditioner");
+    Level &L0 = S->lv[0];
+    if (L0.prec != F64)
+        return fail(XH_ENOTSUP, "xh_solver_smooth: the finest level is fp32 (mg_prec 2)");
+    const xh_role role = L0.pre;
+    if (role.method != XH_CHEBYSHEV && role.method != XH_JACOBI && role.method != XH_HSGS)
+        return fail(XH_ENOTSUP, "xh_solver_smooth: the finest pre-smoother is not a smoother");
+    S->dflag = nullptr;
+    ledger_level(0);
+    for (int k = 0; k < sweeps; ++k)
+        smooth(S, role, L0, L0.cheb_pre, dx, db, false);
+    ledger_level(-1);
+    int rc = check_launch();
+    if (rc)
+        return rc;
+    XH_HIP(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+int xh_solver_profile(xh_solver S, const double *db, double *dx, int32_t iters,
+                      xh_kernel_stat *stats, int32_t max_stats, int32_t *nstats) {
+    if (!S || !db || !dx || iters < 1 || (max_stats > 0 && !stats) || !nstats)
+        return fail(XH_EINVAL, "xh_solver_profile: bad arguments");
+    if (g_ledger)
+        return fail(XH_EINVAL, "xh_solver_profile: a profile solve is already running");
+    // exactly `iters` iterations, launched one by one: no tolerance, no graph
+    const xh_params saved = S->prm;
+    S->prm.graph = 0;
+    S->prm.solver.max_iters = iters;
+    S->prm.solver.abs_tol = 0.0f;
+    S->prm.solver.rel_tol = 0.0f;
+    Ledger L;
+    g_ledger = &L;
+    int rc = solve_impl(S, db, dx);
+    g_ledger = nullptr;
+    S->prm = saved;
+    if (rc)
+        return rc;
+    if (L.failed)
+        return fail(XH_EHIP, "xh_solver_profile: event record failed");
+    XH_HIP(hipStreamSynchronize(S->stream));
+    // per (kernel, level) sums over the launches of the init step and iterations 1..iters
+    // (launches enqueued behind the last iteration found the solve done and did nothing)
+    std::vector<xh_kernel_stat> out;
+    std::unordered_map<std::string, size_t> at;
+    for (const LedgerRec &r : L.recs) {
+        if (r.iter > iters)
+            continue;
+        float ms = 0.0f;
+        XH_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+        const std::string key = r.kernel + "@" + std::to_string(r.level);
+        auto it = at.find(key);
+        if (it == at.end()) {
+            xh_kernel_stat z;
+            std::memset(&z, 0, sizeof(z));
+            std::snprintf(z.kernel, sizeof(z.kernel), "%s", r.kernel.c_str());
+            z.level = r.level;
+            it = at.emplace(key, out.size()).first;
+            out.push_back(z);
+        }
+        xh_kernel_stat &k = out[it->second];
+        k.launches += 1;
+        k.bytes += r.bytes;
+        k.ms += ms;
+    }
+    std::stable_sort(out.begin(), out.end(),
+                     [](const xh_kernel_stat &a, const xh_kernel_stat &b) { return a.ms > b.ms; });
+    *nstats = (int32_t)out.size();
+    for (int i = 0; i < std::min<int>(max_stats, (int)out.size()); ++i)
+        stats[i] = out[i];
+    return 0;
+}
+
 int xh_solver_renew(xh_solver S, const xh_role *solver) {
     if (!S || !solver)
         return fail(XH_EINVAL, "xh_solver_renew: bad arguments");
@@ -1608,6 +1815,40 @@ extern "C" int xh_copy_device(double *dst, const double *src, size_t count) {
     return 0;
 }
 
+extern "C" int xh_copy_device_timed(double *dst, const double *src, size_t count, int reps,
+                                    double *ms_per_launch) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (!dst || !src || reps < 1 || !ms_per_launch)
+        return fail(XH_EINVAL, "xh_copy_device_timed: bad arguments");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XH_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        copy_vec((int64_t)count, F64, src, dst, nullptr, g_stream); // warm-up
+        e = hipEventRecord(e0, g_stream);
+    }
+    for (int i = 0; e == hipSuccess && i < reps; ++i) {
+        copy_vec((int64_t)count, F64, src, dst, nullptr, g_stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipEventRecord(e1, g_stream);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    if (e != hipSuccess)
+        return fail(XH_EHIP, std::string("xh_copy_device_timed: ") + hipGetErrorString(e));
+    *ms_per_launch = (double)ms / reps;
+    return 0;
+}
+
 extern "C" int xh_set_option(const char *key, int64_t value) {
     if (!key)
         return fail(XH_EINVAL, "xh_set_option: null key");
@@ -1622,6 +1863,12 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.fold_min_bytes = value;
     else if (k == "spmv_grid_enable" && (value == 0 || value == 1))
         g_tune.grid_enable = (int)value;
+    else if (k == "spmv_st7_enable" && (value == 0 || value == 1))
+        g_tune.st7_enable = (int)value;
+    else if (k == "spmv_st7_waves" && value >= 8 && value <= (1 << 20))
+        g_tune.st7_waves = (int)value;
+    else if (k == "spmv_st7_min_seg" && value >= 1 && value <= (1 << 24))
+        g_tune.st7_min_seg = (int)value;
     else if (k == "spmv_grid_items" && value >= 1 && value <= (1 << 24))
         g_tune.grid_items = (int)value;
     else if (k == "spmv_grid_min_seg" && value >= 1 && value <= (1 << 24))
@@ -1709,6 +1956,13 @@ extern "C" int xh_grid_schedule(int32_t nx, int32_t ny, int32_t nz, int prec, in
             items[4 * k + 2] = c0;
             items[4 * k + 3] = c1;
         }
+    return 0;
+}
+
+extern "C" int xh_matrix_layouts(xh_matrix A, int32_t *flags) {
+    if (!A || !flags)
+        return fail(XH_EINVAL, "xh_matrix_layouts: bad arguments");
+    *flags = (A->m.gcode ? 1 : 0) | (A->m.st7 ? 2 : 0) | (A->m.idx == IDX_U16 ? 4 : 0);
     return 0;
 }
 

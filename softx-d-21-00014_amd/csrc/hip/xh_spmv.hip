@@ -827,6 +827,208 @@ __global__ __launch_bounds__(kBlock) void k_spmv_grid(
     if constexpr (ND > 0)
         block_partials_in<NV, W, ND>(part, partials, lb, sh.red);
 }
+
+// ---- 7-point stencil SpMV: register window along grid lines ----
+// Matrices whose rows are the points of an nx x ny x nz grid in natural order and whose
+// row i holds exactly the diagonal and its in-grid neighbours -z, -y, -x, +x, +y, +z in
+// that order (diagonal first, then ascending columns: the reference generators' order)
+// carry a second value layout set at upload, st7[i][8] = the row's values in the slots
+// {C, -z, -y, -x, +x, +y, +z, unused} (0 where the neighbour lies outside the grid), and
+// need neither column indices nor row pointers.  A wave owns GY x GZ grid lines (one row
+// of each at a time, L lanes per row) and sweeps them along x: x(c-1) and x(c) stay in
+// registers from the previous steps and only x(c+1) is loaded; a neighbour line inside
+// the wave's block is read from the lanes that own it (cross-lane permute) and only the
+// lines on the block's border are loaded.  For nv = 32 in fp64 (2 x 2 lines) a step of 4
+// rows issues 3 x-row loads where the row-parallel kernel issues 7.  The next step's x
+// rows, values and epilogue operands are loaded while the current step computes.  Each
+// present entry is added in stored order with the same products as the row-parallel
+// kernel: the results are identical.
+
+// lines per wave in y and z for G = 64 / L row groups
+template <int G>
+struct StBlock {
+    static constexpr int GY = G >= 32 ? 8 : G >= 8 ? 4 : G >= 2 ? 2 : 1;
+    static constexpr int GZ = G / GY;
+};
+
+template <typename VT, int W>
+__device__ __forceinline__ Pack<VT, W> shfl_pack(const Pack<VT, W> &v, int src) {
+    Pack<VT, W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        r.v[w] = __shfl(v.v[w], src, 64);
+    return r;
+}
+
+template <typename VT, int NV, int OP>
+__global__ __launch_bounds__(kBlock) void k_spmv_st7(
+    const VT *__restrict__ st7, const StSched g, const VT *__restrict__ x,
+    void *__restrict__ yv, const VT *__restrict__ b, const VT *__restrict__ o1,
+    const VT *__restrict__ isd, const VT *__restrict__ rr, VT *__restrict__ out2,
+    VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last, int x_zero, int out_f32,
+    double *__restrict__ partials, const int *__restrict__ done) {
+    constexpr int ND = NDots<OP>::value;
+    constexpr int W = Width<VT, NV>::value;
+    constexpr int L = NV / W;      // lanes per row
+    constexpr int G = 64 / L;      // rows (lines) per wave
+    constexpr int GY = StBlock<G>::GY, GZ = StBlock<G>::GZ;
+    constexpr int LV = L < 8 ? L : 8;   // lanes of a row that load its 8 value slots
+    constexpr int EV = 8 / LV;          // slots per loading lane
+    using P = Pack<VT, W>;
+    if (done != nullptr && *done)
+        return;
+    const int64_t lb = logical_block();
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / L, a = grp % GY, bz = grp / GY;
+    const int gl = lane % L, k0 = gl * W;
+    const int64_t nwave = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t wid = lb * (kBlock / 64) + (threadIdx.x >> 6);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double part[ND > 0 ? ND : 1][W];
+#pragma unroll
+    for (int d = 0; d < (ND > 0 ? ND : 1); ++d)
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            part[d][w] = 0.0;
+    P zero;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        zero.v[w] = VT(0);
+
+    for (int64_t item = wid; item < g.nitems; item += nwave) {
+        const int yb = (int)(item % g.nyb);
+        const int64_t t = item / g.nyb;
+        const int zb = (int)(t % g.nzb), seg = (int)(t / g.nzb);
+        const int y = yb * GY + a, z = zb * GZ + bz;
+        const int c0 = seg * g.seglen;
+        const int c1 = c0 + g.seglen < nx ? c0 + g.seglen : nx;
+        const bool rowok = y < ny && z < nz;
+        const int64_t line = rowok ? ((int64_t)z * ny + y) * nx : 0;
+        // the border line each row loads in y and z (-1: none), and for a single z line
+        // both z neighbours
+        const bool ylo = GY == 1 || a == 0, yhi = GY == 1 || a == GY - 1;
+        const bool zlo = GZ == 1 || bz == 0, zhi = GZ == 1 || bz == GZ - 1;
+        const bool has_s = y > 0, has_n = y + 1 < ny, has_d = z > 0, has_u = z + 1 < nz;
+        // line offsets (rows) of the loaded border neighbours
+        const int64_t ldy_s = -(int64_t)nx, ldy_n = nx;
+        const int64_t ldz_d = -(int64_t)nx * ny, ldz_u = (int64_t)nx * ny;
+        const bool ld_s = rowok && ylo && has_s, ld_n = rowok && yhi && has_n;
+        const bool ld_d = rowok && zlo && has_d, ld_u = rowok && zhi && has_u;
+        auto xat = [&](int64_t r) __attribute__((always_inline)) {
+            return ld<VT, W>(x + (size_t)r * NV + k0);
+        };
+        // registers of step cc: border neighbours (y: S/N, z: D/U), values, epilogue
+        struct Step {
+            P bs, bn, bd, bu;
+            VT v[EV];
+            EpiIn<VT, W> in;
+        };
+        auto load_step = [&](int cc, Step &S) __attribute__((always_inline)) {
+            const int64_t i = line + cc;
+            S.bs = ld_s ? xat(i + ldy_s) : zero;
+            S.bn = ld_n ? xat(i + ldy_n) : zero;
+            S.bd = ld_d ? xat(i + ldz_d) : zero;
+            S.bu = ld_u ? xat(i + ldz_u) : zero;
+#pragma unroll
+            for (int q = 0; q < EV; ++q)
+                S.v[q] = (rowok && gl < LV) ? st7[(size_t)i * 8 + q * LV + gl] : VT(0);
+            if (rowok) {
+                const size_t o = (size_t)i * NV + k0;
+                epi_load<VT, NV, W, OP>(i, o, o, k0, S.in, yv, b, o1, isd, rr, out2, xupdv, last,
+                                        x_zero);
+            }
+        };
+        P xw = zero, xc = zero, xe = zero;
+        if (rowok) {
+            if (c0 > 0)
+                xw = xat(line + c0 - 1);
+            xc = xat(line + c0);
+            if (c0 + 1 < nx)
+                xe = xat(line + c0 + 1);
+        }
+        Step cur, nxt;
+        load_step(c0, cur);
+        for (int cc = c0; cc < c1; ++cc) {
+            // next step: x(cc + 2), border rows and values of cc + 1
+            const bool more = cc + 1 < c1;
+            P xn = zero;
+            if (rowok && more && cc + 2 < nx)
+                xn = xat(line + cc + 2);
+            if (more)
+                load_step(cc + 1, nxt);
+            // neighbours inside the wave's block from the lanes that own those lines
+            P ys, yn, zd, zu;
+            if constexpr (GY == 1) {
+                ys = cur.bs;
+                yn = cur.bn;
+            } else if constexpr (GY == 2) {
+                const P sib = shfl_pack<VT, W>(xc, lane ^ L);
+                ys = a == 0 ? cur.bs : sib;
+                yn = a == 0 ? sib : cur.bn;
+            } else {
+                const P sl = shfl_pack<VT, W>(xc, lane >= L ? lane - L : lane);
+                const P sh = shfl_pack<VT, W>(xc, lane + L < 64 ? lane + L : lane);
+                ys = a == 0 ? cur.bs : sl;
+                yn = a == GY - 1 ? cur.bn : sh;
+            }
+            if constexpr (GZ == 1) {
+                zd = cur.bd;
+                zu = cur.bu;
+            } else if constexpr (GZ == 2) {
+                const P sib = shfl_pack<VT, W>(xc, lane ^ (GY * L));
+                zd = bz == 0 ? cur.bd : sib;
+                zu = bz == 0 ? sib : cur.bu;
+            } else {
+                const P sl = shfl_pack<VT, W>(xc, lane >= GY * L ? lane - GY * L : lane);
+                const P sh = shfl_pack<VT, W>(xc, lane + GY * L < 64 ? lane + GY * L : lane);
+                zd = bz == 0 ? cur.bd : sl;
+                zu = bz == GZ - 1 ? cur.bu : sh;
+            }
+            VT vs[7];
+            static_for<0, 7>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                if constexpr (L >= 2)
+                    vs[s] = bcast<L, s % LV>(cur.v[s / LV]);
+                else
+                    vs[s] = cur.v[s];
+            });
+            if (rowok) {
+                VT acc[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    acc[w] = OP == OP_ADD ? cur.in.a.v[w] : VT(0);
+                auto add = [&](const P &xv, VT v) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        acc[w] = acc[w] + xv.v[w] * v;
+                };
+                add(xc, vs[0]);
+                if (has_d)
+                    add(zd, vs[1]);
+                if (has_s)
+                    add(ys, vs[2]);
+                if (cc > 0)
+                    add(xw, vs[3]);
+                if (cc + 1 < nx)
+                    add(xe, vs[4]);
+                if (has_n)
+                    add(yn, vs[5]);
+                if (has_u)
+                    add(zu, vs[6]);
+                const size_t o = (size_t)(line + cc) * NV + k0;
+                row_epilogue<VT, NV, W, OP>(o, o, acc, part, cur.in, yv, out2, out3, xupdv, c,
+                                            last, x_zero, out_f32);
+            }
+            xw = xc;
+            xc = xe;
+            xe = xn;
+            if (more)
+                cur = nxt;
+        }
+    }
+    if constexpr (ND > 0)
+        block_partials<NV, W, ND>(part, partials, lb);
+}
 } // namespace
 
 // Tile schedule and grid of one launch: `rpb` rows per workgroup pass, at most `cap`
@@ -892,6 +1094,46 @@ GridLaunch grid_plan(int nx, int ny, int nz, int rpb, int cap) {
     G.grid = (int)std::max<int64_t>(8, (grid + 7) / 8 * 8);
     return G;
 }
+StLaunch st7_plan(int nx, int ny, int nz, int gy, int gz, int cap) {
+    StLaunch S;
+    StSched &g = S.g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.nyb = (ny + gy - 1) / gy;
+    g.nzb = (nz + gz - 1) / gz;
+    const int64_t blocks = (int64_t)g.nyb * g.nzb;
+    // resident waves of one launch (at most one per item, at most `cap` workgroups)
+    const int64_t waves = std::min<int64_t>(std::max(8, g_tune.st7_waves),
+                                            (int64_t)cap * (kBlock / 64));
+    // segments per line: the count (up to 8, segments of at least st7_min_seg rows) whose
+    // items fill the waves' rounds best, the fewest on a tie
+    const int64_t maxseg =
+        std::max<int64_t>(1, std::min<int64_t>(8, nx / std::max(1, g_tune.st7_min_seg)));
+    int64_t nseg = 1;
+    double best = -1.0;
+    for (int64_t q = 1; q <= maxseg; ++q) {
+        const int64_t items = blocks * q;
+        const int64_t w = std::min(items, waves);
+        const int64_t rounds = (items + w - 1) / w;
+        const double eff = (double)items / (double)(rounds * w) * (double)w / (double)waves;
+        if (eff > best + 1e-9) {
+            best = eff;
+            nseg = q;
+        }
+    }
+    g.seglen = (int)((nx + nseg - 1) / nseg);
+    g.nseg = (nx + g.seglen - 1) / g.seglen;
+    g.nitems = blocks * g.nseg;
+    const int64_t nw = std::min<int64_t>(g.nitems, waves);
+    int64_t grid = (nw + (kBlock / 64) - 1) / (kBlock / 64);
+    grid = std::min<int64_t>(grid, cap);
+    S.grid = (int)std::max<int64_t>(8, (grid + 7) / 8 * 8);
+    if (S.grid > cap)
+        S.grid = cap / 8 * 8;
+    return S;
+}
+
 namespace {
 
 template <typename VT, typename IT, int NV, int OP>
@@ -900,6 +1142,19 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
+    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 && NV >= 8) {
+        if (A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr) {
+            constexpr int G = 64 / (NV / W);
+            const StLaunch S =
+                st7_plan(A.gx, A.gy, A.gz, StBlock<G>::GY, StBlock<G>::GZ, cap);
+            hipLaunchKernelGGL((k_spmv_st7<VT, NV, OP>), dim3(S.grid), dim3(kBlock), 0, s,
+                               (const VT *)A.st7, S.g, (const VT *)x, y, (const VT *)a.b,
+                               (const VT *)a.o1, (const VT *)a.isd, (const VT *)a.r,
+                               (VT *)a.out2, (VT *)a.out3, a.xupd, (VT)a.c, a.last, a.x_zero,
+                               a.out_f32, a.partials, a.done);
+            return ND > 0 ? S.grid : 0;
+        }
+    }
     if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32) {
         if (A.gcode != nullptr && g_tune.grid_enable && A.row_ind == nullptr) {
             const GridLaunch G = grid_plan(A.gx, A.gy, A.gz, RPB, cap);
@@ -995,10 +1250,86 @@ int by_nv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArg
 
 } // namespace
 
-int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
-         hipStream_t s) {
-    if (A.nrows == 0)
-        return 0;
+namespace {
+
+const char *op_name(SpmvOp op) {
+    static const char *names[] = {"OP_SET",  "OP_ADD",       "OP_RESID",    "OP_DOT1",
+                                  "OP_DOT3", "OP_CHEB0",     "OP_CHEB",     "OP_PROLONG32",
+                                  "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4"};
+    return (op >= 0 && op <= OP_RDOT4) ? names[op] : "OP_?";
+}
+
+int op_ndots(SpmvOp op) {
+    return (op == OP_DOT1 || op == OP_RESID0) ? 1 : op == OP_DOT3 ? 3 : op == OP_RDOT4 ? 4 : 0;
+}
+
+// the launch_t conditions for the 7-point stencil and the structured-grid kernels
+bool takes_st7(const Mat &A, SpmvOp op, int nv) {
+    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && nv >= 8 &&
+           A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr;
+}
+bool takes_grid(const Mat &A, SpmvOp op) {
+    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && A.gcode != nullptr &&
+           g_tune.grid_enable && A.row_ind == nullptr;
+}
+
+// Algorithmic HBM bytes of one SpMV launch: the matrix streams once (values, column
+// indices or grid codes, row pointers, panel bases, row map), x once (ncols rows), and
+// the epilogue's vector operands and results once each; dot partials are added by the
+// caller from the launch's partial count.
+double spmv_bytes(const Mat &A, int nv, SpmvOp op, const SpmvArgs &a, bool grid, bool st7) {
+    const double sv = A.prec == F64 ? 8.0 : 4.0;
+    const double si = grid ? 1.0 : A.idx == IDX_U32 ? 4.0 : A.idx == IDX_U16 ? 2.0 : 1.0;
+    const double n = (double)A.nrows, V = n * nv * sv;
+    double m = (double)A.nnz * (sv + si) + (n + 1.0) * 4.0;
+    if (st7)
+        m = n * 8.0 * sv; // the 8 value slots of every row, nothing else
+    if (A.idx == IDX_U16 && !grid && !st7)
+        m += (double)(((A.nrows - 1) >> A.panel_shift) + 1) * 4.0;
+    if (A.row_ind != nullptr)
+        m += n * 4.0;
+    const double xb = (double)A.ncols * nv * sv;
+    double vec = 0.0;
+    switch (op) {
+    case OP_SET:
+        vec = V; // y
+        break;
+    case OP_ADD:
+        vec = 2.0 * V; // y read and written
+        break;
+    case OP_RESID:
+        vec = V + n * nv * (a.out_f32 ? 4.0 : sv); // b ; y
+        break;
+    case OP_OFFBLOCK:
+    case OP_DOT1:
+    case OP_DOT3:
+        vec = 2.0 * V; // b or o1 ; y
+        break;
+    case OP_RESID0:
+        vec = 4.0 * V; // b ; y, out2, out3
+        break;
+    case OP_RDOT4:
+        vec = 5.0 * V; // v, r0, r ; t, r^
+        break;
+    case OP_CHEB0:
+        // b, isd ; last: x (the operand) written, else r and u written
+        vec = V + n * sv + (a.last ? V : 2.0 * V);
+        break;
+    case OP_CHEB:
+        // r, isd ; last: x written (and read unless zero-flagged), else u' written
+        vec = V + n * sv + (a.last ? (a.x_zero ? V : 2.0 * V) : V);
+        break;
+    case OP_PROLONG32:
+        vec = n * nv * 8.0 * (a.x_zero ? 1.0 : 2.0); // fp64 x written (and read)
+        break;
+    default:
+        break;
+    }
+    return m + xb + vec;
+}
+
+int spmv_dispatch(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
+                  hipStream_t s) {
     if (A.prec == F64) {
         if (A.idx == IDX_U8)
             return by_nv<double, uint8_t>(A, nv, x, y, op, a, s);
@@ -1009,6 +1340,33 @@ int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs
         return by_nv<float, uint8_t>(A, nv, x, y, op, a, s);
     return A.idx == IDX_U16 ? by_nv<float, uint16_t>(A, nv, x, y, op, a, s)
                             : by_nv<float, uint32_t>(A, nv, x, y, op, a, s);
+}
+
+} // namespace
+
+const char *prec_name(int prec) { return prec == F64 ? "double" : "float"; }
+const char *idx_name(int idx) {
+    return idx == IDX_U32 ? "u32" : idx == IDX_U16 ? "u16" : "u8";
+}
+
+int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
+         hipStream_t s) {
+    if (A.nrows == 0)
+        return 0;
+    LedgerScope led(s);
+    const int np = spmv_dispatch(A, nv, x, y, op, a, s);
+    if (led.on) {
+        const bool st7 = takes_st7(A, op, nv);
+        const bool grid = !st7 && takes_grid(A, op);
+        const std::string tail = std::to_string(nv) + "," + op_name(op) + ">";
+        led.kernel = st7    ? std::string("k_spmv_st7<") + prec_name(A.prec) + "," + tail
+                     : grid ? std::string("k_spmv_grid<") + prec_name(A.prec) + "," + tail
+                            : std::string("k_spmv<") + prec_name(A.prec) + "," +
+                                  idx_name(A.idx) + "," + tail;
+        led.bytes = spmv_bytes(A, nv, op, a, grid, st7) +
+                    (np > 0 ? (double)np * op_ndots(op) * nv * 8.0 : 0.0);
+    }
+    return np;
 }
 
 } // namespace xh

@@ -496,6 +496,18 @@ __global__ __launch_bounds__(kBlock) void k_bumps(int nx, int ny, int nz, int nv
     }
 }
 
+// x = u + x (the axpby(a1, t1, a1, x) that ends a Chebyshev application,
+// chebyshev.inl:201-202)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_add(int64_t count, const T *__restrict__ u,
+                                                T *__restrict__ x, const int *__restrict__ done) {
+    if (done != nullptr && *done)
+        return;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * kBlock)
+        x[e] = u[e] + x[e];
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill0(int64_t count, T *__restrict__ b,
                                                   const int *__restrict__ done) {
@@ -513,8 +525,8 @@ int grid_for(int64_t count) {
 }
 
 template <typename VT, typename IT, int NV>
-void hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int64_t B,
-            const int *done, hipStream_t s) {
+const char *hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int64_t B,
+                   const int *done, hipStream_t s) {
     constexpr int G = 64 / NV;
     const int64_t nblk = (A.nrows + B - 1) / B;
     const int grid = (int)((nblk + G - 1) / G);
@@ -528,7 +540,7 @@ void hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int6
                                (const uint32_t *)A.col, (const VT *)A.val,
                                (const VT *)A.inv_diag, A.nrows, B, A.core_emax, gb,
                                (const VT *)t, (VT *)x, x_zero, (VT)relax, done);
-            return;
+            return "k_hsgs_core";
         }
     }
     if (lds <= 64 * 1024) {
@@ -536,16 +548,17 @@ void hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double relax, int6
                            (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
                            (const VT *)A.inv_diag, A.nrows, B, (const VT *)t, (VT *)x, x_zero,
                            (VT)relax, done);
-    } else {
-        hipLaunchKernelGGL((k_hsgs_glb<VT, IT, NV>), dim3(grid), dim3(64), 0, s, A.row,
-                           (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
-                           (const VT *)A.inv_diag, A.nrows, B, (const VT *)t, (VT *)x, x_zero,
-                           (VT)relax, done);
+        return "k_hsgs_lds";
     }
+    hipLaunchKernelGGL((k_hsgs_glb<VT, IT, NV>), dim3(grid), dim3(64), 0, s, A.row,
+                       (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
+                       (const VT *)A.inv_diag, A.nrows, B, (const VT *)t, (VT *)x, x_zero,
+                       (VT)relax, done);
+    return "k_hsgs_glb";
 }
 
 template <typename VT, typename IT>
-void hsgs_nv(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax, int64_t B,
+const char *hsgs_nv(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax, int64_t B,
              const int *done, hipStream_t s) {
     switch (nv) {
     case 1:
@@ -563,13 +576,34 @@ void hsgs_nv(const Mat &A, int nv, const void *t, void *x, int x_zero, double re
     case 64:
         return hsgs_t<VT, IT, 64>(A, t, x, x_zero, relax, B, done, s);
     default:
-        return;
+        return nullptr;
     }
 }
 
 } // namespace
 
-int vec_op(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s) {
+namespace {
+
+// vectors of n x nv doubles each vector op reads plus writes, and its dot count
+int vec_streams(VecOp op) {
+    static const int rw[] = {3, 8, 4, 5, 4, 4, 7, 2, 2, 6, 2, 3, 3, 2, 11, 11};
+    return (op >= 0 && op <= V_RB_UPDATE_MERGED) ? rw[op] : 0;
+}
+int vec_ndots(VecOp op) {
+    return (op == V_XR_UPDATE || op == V_R_UPDATE || op == V_DOT1 || op == V_COPY_DOT) ? 1
+           : op == V_DOT2                                                             ? 2
+           : op == V_DOT3                                                             ? 3
+                                                                                      : 0;
+}
+const char *vec_name(VecOp op) {
+    static const char *names[] = {"V_S_UPDATE", "V_XR_UPDATE", "V_P_UPDATE", "V_GROUP1",
+                                  "V_X_UPDATE", "V_R_UPDATE", "V_GROUP5", "V_DOT3", "V_DOT1",
+                                  "V_CG_XR", "V_DOT2", "V_CG_P", "V_COPY_DOT", "V_COPY",
+                                  "V_RB_UPDATE", "V_RB_UPDATE_MERGED"};
+    return (op >= 0 && op <= V_RB_UPDATE_MERGED) ? names[op] : "V_?";
+}
+
+int vec_dispatch(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s) {
     const int gate = a.gate;
     const VecArgs &b = a;
     switch (nv) {
@@ -592,14 +626,38 @@ int vec_op(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s) {
     }
 }
 
+} // namespace
+
+int vec_op(VecOp op, int64_t n, int nv, const VecArgs &a, hipStream_t s) {
+    LedgerScope led(s);
+    const int np = vec_dispatch(op, n, nv, a, s);
+    if (led.on) {
+        led.kernel = std::string("k_vec<") + std::to_string(nv) + "," + vec_name(op) + ">";
+        led.bytes = (double)vec_streams(op) * n * nv * 8.0 +
+                    (np > 0 ? (double)np * vec_ndots(op) * nv * 8.0 : 0.0);
+    }
+    return np;
+}
+
 void finalize(FinOp op, SolverState *st, const double *partials, int nparts, int nv,
               hipStream_t s) {
+    LedgerScope led(s);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, (int)op, st, partials, nparts, nv);
+    if (led.on) {
+        led.kernel = "k_finalize";
+        led.bytes = (double)nparts * fin_ndots(op) * nv * 8.0;
+    }
 }
 
 void cheb_init_zero(int64_t n, int nv, int prec, const void *b, const void *isd, double c,
                     void *r, void *u_or_x, int last, const int *done, hipStream_t s) {
     const int64_t count = n * nv;
+    LedgerScope led(s);
+    if (led.on) {
+        const double sv = prec == F64 ? 8.0 : 4.0;
+        led.kernel = std::string("k_cheb_init_zero<") + prec_name(prec) + ">";
+        led.bytes = sv * (double)(count * (last ? 2 : 3) + n); // b, isd ; u or x (and r)
+    }
     if (prec == F64)
         hipLaunchKernelGGL(k_cheb_init_zero<double>, dim3(grid_for(count)), dim3(kBlock), 0, s,
                            count, nv, (const double *)b, (const double *)isd, c, (double *)r,
@@ -614,6 +672,13 @@ void jacobi_update(int64_t n, int nv, int prec, void *x, const void *b, const vo
                    const void *inv_diag, double relax, int x_zero, const int *done,
                    hipStream_t s) {
     const int64_t count = n * nv;
+    LedgerScope led(s);
+    if (led.on) {
+        const double sv = prec == F64 ? 8.0 : 4.0;
+        led.kernel = std::string("k_jacobi<") + prec_name(prec) + ">";
+        // b, inverse diagonal ; x written (x and r read unless x is zero-flagged)
+        led.bytes = sv * (double)(count * (x_zero ? 2 : 4) + n);
+    }
     if (prec == F64)
         hipLaunchKernelGGL(k_jacobi<double>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
                            nv, (double *)x, (const double *)b, (const double *)r,
@@ -633,16 +698,22 @@ int64_t hsgs_block_rows(int64_t n, int nblocks) {
 
 void hsgs_sweeps(const Mat &A, int nv, const void *t, void *x, int x_zero, double relax,
                  int64_t B, const int *done, hipStream_t s) {
-    if (A.prec == F64) {
-        if (A.idx == IDX_U16)
-            hsgs_nv<double, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s);
-        else
-            hsgs_nv<double, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
-    } else {
-        if (A.idx == IDX_U16)
-            hsgs_nv<float, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s);
-        else
-            hsgs_nv<float, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
+    LedgerScope led(s);
+    const char *name;
+    if (A.prec == F64)
+        name = A.idx == IDX_U16 ? hsgs_nv<double, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s)
+                                : hsgs_nv<double, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
+    else
+        name = A.idx == IDX_U16 ? hsgs_nv<float, uint16_t>(A, nv, t, x, x_zero, relax, B, done, s)
+                                : hsgs_nv<float, uint32_t>(A, nv, t, x, x_zero, relax, B, done, s);
+    if (led.on && name) {
+        // the swept matrix (core blocks or the whole operator), its inverse diagonal, t ;
+        // x written (and read unless zero-flagged)
+        const double sv = A.prec == F64 ? 8.0 : 4.0, si = A.idx == IDX_U16 ? 2.0 : 4.0;
+        const double V = (double)A.nrows * nv * sv;
+        led.kernel = std::string(name) + "<" + prec_name(A.prec) + "," + std::to_string(nv) + ">";
+        led.bytes = (double)A.nnz * (sv + si) + (double)(A.nrows + 1) * 4.0 +
+                    (double)A.nrows * sv + V + (x_zero ? V : 2.0 * V);
     }
 }
 
@@ -650,6 +721,12 @@ void dense_matvec(int64_t n, int nv, int prec, const void *D, const void *x, voi
                   const int *done, hipStream_t s) {
     const int64_t count = n * nv;
     const int grid = (int)((count + kBlock - 1) / kBlock);
+    LedgerScope led(s);
+    if (led.on) {
+        const double sv = prec == F64 ? 8.0 : 4.0;
+        led.kernel = std::string("k_dense<") + prec_name(prec) + ">";
+        led.bytes = sv * (double)(n * n + 2 * count); // D, x, y
+    }
     if (prec == F64)
         hipLaunchKernelGGL(k_dense<double>, dim3(grid), dim3(kBlock), 0, s, n, nv,
                            (const double *)D, (const double *)x, (double *)y, done);
@@ -660,6 +737,11 @@ void dense_matvec(int64_t n, int nv, int prec, const void *D, const void *x, voi
 
 void convert(int64_t count, int from_prec, const void *src, void *dst, const int *done,
              hipStream_t s) {
+    LedgerScope led(s);
+    if (led.on) {
+        led.kernel = from_prec == F64 ? "k_convert<double,float>" : "k_convert<float,double>";
+        led.bytes = 12.0 * (double)count;
+    }
     if (from_prec == F64)
         hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(count)), dim3(kBlock), 0, s,
                            count, (const double *)src, (float *)dst, done);
@@ -668,8 +750,27 @@ void convert(int64_t count, int from_prec, const void *src, void *dst, const int
                            count, (const float *)src, (double *)dst, done);
 }
 
+void add_vec(int64_t count, int prec, const void *u, void *x, const int *done, hipStream_t s) {
+    LedgerScope led(s);
+    if (led.on) {
+        led.kernel = std::string("k_add<") + prec_name(prec) + ">";
+        led.bytes = (prec == F64 ? 24.0 : 12.0) * (double)count;
+    }
+    if (prec == F64)
+        hipLaunchKernelGGL(k_add<double>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           (const double *)u, (double *)x, done);
+    else
+        hipLaunchKernelGGL(k_add<float>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
+                           (const float *)u, (float *)x, done);
+}
+
 void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *done,
               hipStream_t s) {
+    LedgerScope led(s);
+    if (led.on) {
+        led.kernel = std::string("k_convert<") + prec_name(prec) + "," + prec_name(prec) + ">";
+        led.bytes = (prec == F64 ? 16.0 : 8.0) * (double)count;
+    }
     if (prec == F64)
         hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(count)), dim3(kBlock), 0,
                            s, count, (const double *)src, (double *)dst, done);
@@ -699,6 +800,11 @@ void eval_bumps(int nx, int ny, int nz, int nv, int maxb, const int *count, cons
 }
 
 void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s) {
+    LedgerScope led(s);
+    if (led.on) {
+        led.kernel = std::string("k_fill0<") + prec_name(prec) + ">";
+        led.bytes = (prec == F64 ? 8.0 : 4.0) * (double)count;
+    }
     if (prec == F64)
         hipLaunchKernelGGL(k_fill0<double>, dim3(grid_for(count)), dim3(kBlock), 0, s, count,
                            (double *)dst, done);

@@ -45,6 +45,11 @@ class Params(C.Structure):
                 ("post_lev", Role * MAX_LEVELS)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("kernel", C.c_char * 80), ("level", C.c_int32), ("launches", C.c_int32),
+                ("bytes", C.c_double), ("ms", C.c_double)]
+
+
 class AmgParams(C.Structure):
     _fields_ = [("strength", C.c_float), ("max_row_sum", C.c_float), ("coarse_size", C.c_int32),
                 ("max_levels", C.c_int32), ("trunc_factor", C.c_float), ("pmax", C.c_int32),
@@ -113,6 +118,7 @@ def _load():
         "xh_grid_schedule": [C.c_int32, C.c_int32, C.c_int32, C.c_int, C.c_int, C.c_int,
                              P(i64), P(i64), P(i64), vp],
         "xh_matrix_grid": [vp, vp],
+        "xh_matrix_layouts": [vp, P(i32)],
         "xh_spmv_schedule": [i64, i64, i64, C.c_int, C.c_int, C.c_int, P(i64), P(i64), P(i64),
                              vp],
         "xh_set_option": [C.c_char_p, i64],
@@ -150,6 +156,9 @@ def _load():
         "xh_solver_stats": [vp, P(i32), vp, vp, vp],
         "xh_solver_history": [vp, vp, i32, P(i32)],
         "xh_solver_precond_apply": [vp, vp, vp],
+        "xh_solver_profile": [vp, vp, vp, i32, vp, i32, P(i32)],
+        "xh_solver_smooth": [vp, vp, vp, i32],
+        "xh_copy_device_timed": [vp, vp, C.c_size_t, C.c_int, P(f64)],
         "xh_solver_renew": [vp, P(Role)],
         "xh_solver_info": [vp, P(C.c_int), P(C.c_size_t)],
         "xh_solver_destroy": [vp],
@@ -381,6 +390,31 @@ def grid_schedule(nx, ny, nz, nv, prec=0, dots=False):
     return g.value, ipw.value, items
 
 
+HBM_PEAK = 8.0e12  # MI355X published HBM3E bandwidth, bytes/s
+
+
+def kernel_table(stats, peak=HBM_PEAK):
+    """rows of a Solver.profile result with per-launch bytes and microseconds, achieved
+    algorithmic GB/s and its fraction of `peak` (bytes/s)"""
+    rows = []
+    for k in stats:
+        gbs = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+        rows.append(dict(kernel=k["kernel"], level=k["level"], launches=k["launches"],
+                         mb_per_launch=round(k["bytes"] / k["launches"] / 1e6, 3),
+                         us_per_launch=round(k["ms"] * 1e3 / k["launches"], 2),
+                         ms_total=round(k["ms"], 4), gb_s=round(gbs, 1),
+                         frac=round(gbs * 1e9 / peak, 4)))
+    return rows
+
+
+def copy_timed(dst, src, count, reps=20):
+    """milliseconds per launch of the library's fp64 copy kernel over count elements"""
+    ms = C.c_double()
+    check(hip().xh_copy_device_timed(dst.ptr, src.ptr, count, reps, C.byref(ms)),
+          "xh_copy_device_timed")
+    return ms.value
+
+
 def set_option(key, value):
     check(hip().xh_set_option(key.encode(), int(value)), "xh_set_option")
 
@@ -541,6 +575,21 @@ class Solver:
 
     def precond_apply(self, db, dx):
         check(hip().xh_solver_precond_apply(self.s, db.ptr, dx.ptr), "xh_solver_precond_apply")
+
+    def smooth(self, db, dx, sweeps=1):
+        """dx <- the finest level's pre-smoother applied `sweeps` times from dx"""
+        check(hip().xh_solver_smooth(self.s, db.ptr, dx.ptr, sweeps), "xh_solver_smooth")
+
+    def profile(self, db, dx, iters, max_stats=512):
+        """per-(kernel, level) launches, algorithmic bytes and event-timed ms of the init
+        step plus `iters` iterations from x = dx (xh_solver_profile); list of dicts,
+        slowest first"""
+        arr = (KernelStat * max_stats)()
+        n = C.c_int32()
+        check(hip().xh_solver_profile(self.s, db.ptr, dx.ptr, iters, arr, max_stats,
+                                      C.byref(n)), "xh_solver_profile")
+        return [dict(kernel=k.kernel.decode(), level=k.level, launches=k.launches,
+                     bytes=k.bytes, ms=k.ms) for k in arr[: min(n.value, max_stats)]]
 
     def stats(self):
         it = C.c_int32()
@@ -797,6 +846,12 @@ class Matrix:
         d = np.zeros(3, np.int32)
         check(hip().xh_matrix_grid(self.m, _ptr(d)), "xh_matrix_grid")
         return tuple(int(v) for v in d) if d[0] else None
+
+    def layouts(self):
+        """set of the device layouts the matrix carries ("grid", "st7", "u16")"""
+        f = C.c_int32()
+        check(hip().xh_matrix_layouts(self.m, C.byref(f)), "xh_matrix_layouts")
+        return {n for b, n in ((1, "grid"), (2, "st7"), (4, "u16")) if f.value & b}
 
     def spmv_timed(self, nv, dx, dy, reps, accumulate=False):
         ms = C.c_double()

@@ -140,7 +140,10 @@ def test_mixed_precision_parity(gpu, O, mg_prec):
     assert rel_diff(g["x"], x64, nv) <= 1e-6
 
 
-@pytest.mark.parametrize("smoother,k", [("Chebyshev", 4), ("HSGS", 1), ("Jacobi", 2)])
+# Chebyshev order 1 post-smoothing updates a non-zero x from r = b - A x: the x update
+# runs after the SpMV (an in-place update in the same launch raced with other rows' reads)
+@pytest.mark.parametrize("smoother,k", [("Chebyshev", 4), ("Chebyshev", 1), ("HSGS", 1),
+                                        ("Jacobi", 2)])
 @pytest.mark.parametrize("mg_prec", [0, 1, 2])
 def test_preconditioner_bit_exact(gpu, O, smoother, k, mg_prec):
     X = gpu
