@@ -187,6 +187,15 @@ int xh_matrix_gather(xh_matrix A, int nv, const void *dx_full, void *dx_c);
  * xamg_rhs_bumps) */
 int xh_eval_bumps(int nx, int ny, int nz, int nv, int max_bumps, const int32_t *count,
                   const double *params, double *db);
+/* xh_eval_bumps without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  The counts and bump parameters are staged in a static device array of
+ * 164,096 bytes per GPU (held until the process ends) instead of hipMalloc'd buffers, so
+ * nv and max_bumps must be at most 64 (XH_EINVAL otherwise).  Same results as
+ * xh_eval_bumps.  All calls on one GPU share the staging array: calls that overlap on one
+ * GPU (several host threads) race. */
+int xh_eval_bumps_noalloc(int nx, int ny, int nz, int nv, int max_bumps,
+                          const int32_t *count, const double *params, double *db);
 
 /* y = D x, D dense row-major n x n (backend::dense_Ax_y, dense_matvec.inl:72-121) */
 int xh_dense_matvec(int64_t n, int nv, int prec, const void *dD, const void *dx, void *dy);

@@ -26,12 +26,26 @@
 #include <unordered_map>
 #include <vector>
 
+// Staging area of xh_eval_bumps_noalloc: the per-RHS bump counts (int32) in the first
+// kScratchCountBytes bytes, then the bump parameters (5 doubles per bump, nv * max_bumps
+// bumps, nv and max_bumps at most 64).  Every GPU the library runs on holds one copy until
+// the process ends; host code reaches it only through the per-device address that xh_init
+// looks up (g_scratch), never through this name.
+constexpr int kScratchMaxNv = 64;
+constexpr int kScratchMaxBumps = 64;
+constexpr size_t kScratchCountBytes = 256;
+constexpr size_t kScratchBytes =
+    kScratchCountBytes + (size_t)kScratchMaxNv * kScratchMaxBumps * 5 * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace xh {
 namespace {
 
 thread_local std::string g_err;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+// device address of `scratch` on each device, set by xh_init with that device current
+std::vector<void *> g_scratch;
 
 int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -1179,6 +1193,13 @@ int xh_init(int device) {
     XH_HIP(hipSetDevice(device));
     if (g_stream == nullptr)
         XH_HIP(hipStreamCreate(&g_stream));
+    if ((int)g_scratch.size() < n)
+        g_scratch.resize(n, nullptr);
+    if (g_scratch[device] == nullptr) {
+        void *p = nullptr;
+        XH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = p;
+    }
     g_device = device;
     return 0;
 }
@@ -2693,5 +2714,36 @@ extern "C" int xh_eval_bumps(int nx, int ny, int nz, int nv, int max_bumps,
     (void)hipFree(dp);
     if (e != hipSuccess)
         return fail(XH_EHIP, std::string("xh_eval_bumps: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int xh_eval_bumps_noalloc(int nx, int ny, int nz, int nv, int max_bumps,
+                                     const int32_t *count, const double *params, double *db) {
+    int rc = ensure_init();
+    if (rc)
+        return rc;
+    if (nx < 1 || ny < 1 || nz < 1 || nv < 1 || max_bumps < 1 || !count || !params || !db)
+        return fail(XH_EINVAL, "xh_eval_bumps_noalloc: bad arguments");
+    if (nv > kScratchMaxNv || max_bumps > kScratchMaxBumps)
+        return fail(XH_EINVAL, "xh_eval_bumps_noalloc: nv and max_bumps must be at most 64 "
+                               "(the library's static staging area)");
+    for (int k = 0; k < nv; ++k)
+        if (count[k] < 0 || count[k] > max_bumps)
+            return fail(XH_EINVAL, "xh_eval_bumps_noalloc: count out of range");
+    unsigned char *base = (unsigned char *)g_scratch[g_device];
+    if (base == nullptr)
+        return fail(XH_EHIP, "xh_eval_bumps_noalloc: staging area not mapped on this device");
+    int *dc = (int *)base;
+    double *dp = (double *)(base + kScratchCountBytes);
+    // every byte the kernel reads (counts of RHS < nv, parameters of bumps < count) is
+    // written here first, so nothing left by an earlier call is seen
+    XH_HIP(hipMemcpyAsync(dc, count, (size_t)nv * sizeof(int), hipMemcpyHostToDevice, g_stream));
+    XH_HIP(hipMemcpyAsync(dp, params, (size_t)nv * max_bumps * 5 * sizeof(double),
+                          hipMemcpyHostToDevice, g_stream));
+    eval_bumps(nx, ny, nz, nv, max_bumps, dc, dp, db, g_stream);
+    rc = check_launch();
+    if (rc)
+        return rc;
+    XH_HIP(hipStreamSynchronize(g_stream));
     return 0;
 }

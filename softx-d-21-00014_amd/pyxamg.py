@@ -147,6 +147,7 @@ def _load():
         "xh_unet_infer": [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp],
         "xh_unet_mfma_selftest": [vp, vp, vp],
         "xh_eval_bumps": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp],
+        "xh_eval_bumps_noalloc": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp],
         "xh_hier_create": [C.c_int, P(vp)],
         "xh_hier_set_level": [vp, C.c_int, i64, i64, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp,
                               vp, vp, f64, f64],
