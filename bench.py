@@ -131,6 +131,8 @@ def learned_x0(X, args):
         dx0 = X.DeviceArray(V * nv)
         c = time.perf_counter()
         net.infer(N, N, N, nv, db, dx0)
+        if args.learned_smooth > 0:
+            Sa.smooth(db, dx0, args.learned_smooth)
         X.hip().xh_synchronize()
         d = time.perf_counter()
         r0_net.append(rel_res(db, dx0))
@@ -156,6 +158,8 @@ def learned_x0(X, args):
         dxl = X.DeviceArray(V * nv)
         h = time.perf_counter()
         net.infer(N, N, N, nv, db, dxl)
+        if args.learned_smooth > 0:
+            Sa.smooth(db, dxl, args.learned_smooth)
         Sm.solve_device(db, dxl)
         X.hip().xh_synchronize()
         k = time.perf_counter()
@@ -187,6 +191,7 @@ def learned_x0(X, args):
         "ms_per_rhs_x0_zero": round(t0s * 1e3 / nr, 4),
         "ms_per_rhs_x0_learned_incl_inference": round(t1s * 1e3 / nr, 4),
         "inference_ms_per_rhs": round(tinf * 1e3 / nr, 4),
+        "x0_smoothing_sweeps": args.learned_smooth,
         "time_to_tolerance_speedup": round(t0s / t1s, 3) if t1s > 0 else None,
         "max_rel_diff_learned_vs_zero_start": max(diffs),
         "iters_x0_zero_fp32_precond": float(np.mean(it2)),
@@ -214,6 +219,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-pairs", type=int, default=2000)
     ap.add_argument("--learned-steps", type=int, default=600)
+    ap.add_argument("--batch-single-launch", action="store_true",
+                    help="also time the batch as one cooperative launch per solve")
+    ap.add_argument("--learned-smooth", type=int, default=0,
+                    help="finest-level smoother sweeps applied to the network's x0")
     ap.add_argument("--cpu-cores", type=int, default=16,
                     help="host threads of the oracle CPU baseline (the CPU share of one GPU)")
     args = ap.parse_args()
@@ -271,6 +280,28 @@ def main():
 
     true_rel = true_rel_res(dx)
     x64 = dx.to_host().reshape(-1, nv)
+
+    # the same system at nv = 8 (the other RHS count of this configuration), fp64
+    nv8 = None
+    if nv != 8:
+        S8 = X.Solver(H, prm, 8)
+        b8 = X.rhs_uniform(1, n, 8)
+        db8, dx8 = X.DeviceArray.from_host(b8), X.DeviceArray(n * 8)
+        for _ in range(args.warmup):
+            dx8.zero()
+            S8.solve_device(db8, dx8)
+        X.hip().xh_synchronize()
+        e0 = time.perf_counter()
+        for _ in range(args.steps):
+            dx8.zero()
+            S8.solve_device(db8, dx8)
+        X.hip().xh_synchronize()
+        e1 = time.perf_counter()
+        it8, conv8, _, rel8 = S8.stats()
+        nv8 = {"ms_per_rhs": round((e1 - e0) * 1e3 / args.steps / 8, 4),
+               "ms_per_step": round((e1 - e0) * 1e3 / args.steps, 3), "iters": it8,
+               "converged": int(conv8.sum()), "max_rel_res": float(rel8.max())}
+        del S8, db8, dx8
 
     # second run of the same workload: FP32 hierarchy, smoothers and SpMM inside the
     # preconditioner, FP64 Krylov vectors, dots and coarsest solve
@@ -334,6 +365,27 @@ def main():
         st_ = [Bt.stats(i) for i in range(64)]
         lat = np.array([q["latency_ms"] for q in st_])
         ms_b = (b1 - b0) * 1e3 / args.steps
+        single = None
+        if args.batch_single_launch:
+            # the same batch as one cooperative launch per solve (device-side iterations)
+            xg = [Bt.x(i) for i in range(64)]
+            Bt.set_persistent(True)
+            for _ in range(args.warmup):
+                Bt.solve()
+            p0 = time.perf_counter()
+            for _ in range(args.steps):
+                Bt.solve()
+            p1 = time.perf_counter()
+            latp = np.array([Bt.stats(i)["latency_ms"] for i in range(64)])
+            ms_p = (p1 - p0) * 1e3 / args.steps
+            single = {
+                "ms_per_batch": round(ms_p, 3),
+                "systems_per_s": round(64 / (ms_p * 1e-3), 2),
+                "p50_latency_ms": round(float(np.percentile(latp, 50)), 3),
+                "p99_latency_ms": round(float(np.percentile(latp, 99)), 3),
+                "identical_to_per_iteration_launches":
+                    bool(all(np.array_equal(Bt.x(i), xg[i]) for i in range(64))),
+            }
         batched = {
             "workload": "64 independent 7-point variable-coefficient systems (k lognormal, "
                         "sigma 1), edge uniform in {20..64}, nv uniform in {1,2,4,8,16}, "
@@ -349,6 +401,7 @@ def main():
             "iters_min_max": [min(q["iters"] for q in st_), max(q["iters"] for q in st_)],
             "all_converged": bool(all(q["conv"].all() for q in st_)),
             "tile_passes_nv_1_2_4_8_16": [int(t) for t in tiles],
+            "single_launch": single,
         }
         del Bt
 
@@ -409,6 +462,7 @@ def main():
         "max_rel_res": float(rel_res.max()),
         "true_rel_res": true_rel,
         "setup_s": round(setup_s, 2),
+        "nv8": nv8,
         "mixed_precision": mixed,
         "batched": batched,
         "learned_x0": learned,

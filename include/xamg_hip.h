@@ -147,7 +147,9 @@ int xh_csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_
 int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv, int dots,
                      int64_t *grid, int64_t *ntiles, int64_t *tpw, int64_t *rows);
 /* Launch-shape knobs: "spmv_strip_budget" (bytes, default 8 MiB), "spmv_tile_passes"
- * (default 8), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
+ * (default 8), "spmv_min_passes" (default 1: in row order, at least this many passes of
+ * rows per workgroup), "dot_groups" (default 1024, a multiple of 8 up to 4096: workgroups
+ * of a launch that writes dot-product partials), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
  * when 2*fold*nv*sizeof(value) exceeds it), "spmv_grid_enable" (0/1, default 0: the
  * structured-grid SpMV of matrices with a grid view), "spmv_grid_items" (default 2048),
  * "spmv_grid_min_seg" (default 32), "unet_fuse_up" (0/1, default 1: U-Net inference reads
@@ -330,6 +332,13 @@ int xh_batch_stats(xh_batch B, int s, int32_t *iters, int32_t *conv, double *abs
  * tile_passes (nullable) receives the 5 choices.  xh_batch_set_tiles sets them. */
 int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes);
 int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes);
+/* on = 1: every later xh_batch_solve is one cooperative (persistent) launch: the init step
+ * and the iterations run on the device, the launch sequence of an iteration as phases
+ * separated by grid-wide barriers, until every system is done or max_iters; the host
+ * issues one launch and waits once.  Same work items and arithmetic as the per-iteration
+ * launches: identical results.  on = 0 (default): one launch sequence (or hipGraph) per
+ * iteration, the host polling the all-done flag. */
+int xh_batch_set_persistent(xh_batch B, int on);
 int xh_batch_destroy(xh_batch B);
 
 #ifdef __cplusplus

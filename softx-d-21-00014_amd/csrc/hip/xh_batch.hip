@@ -36,15 +36,18 @@ __device__ __forceinline__ void item_partials(const double (&part)[3], int nd, i
     __syncthreads();
 }
 
+// one batched operation over the work items wg, wg + nwg, ... (one item per workgroup at
+// a time)
 template <int OP>
-__global__ __launch_bounds__(kBlock) void k_batch(BatchDev d, const BatchItem *__restrict__ items,
-                                                  int nitems, BatchVecs v, int ci, int last,
-                                                  double *__restrict__ partials) {
+__device__ __forceinline__ void batch_phase(const BatchDev &d,
+                                            const BatchItem *__restrict__ items, int nitems,
+                                            const BatchVecs &v, int ci, int last,
+                                            double *__restrict__ partials, int wg, int nwg) {
     constexpr int ND = (OP == B_RESID0 || OP == B_DOT1 || OP == B_XR_UPDATE) ? 1
                        : (OP == B_DOT3)                                      ? 3
                                                                              : 0;
     const int tid = threadIdx.x;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (int it = wg; it < nitems; it += nwg) {
         const BatchItem w = items[it];
         const SolverState *st = d.st + w.sys;
         if (st->done)
@@ -134,6 +137,13 @@ __global__ __launch_bounds__(kBlock) void k_batch(BatchDev d, const BatchItem *_
     }
 }
 
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_batch(BatchDev d, const BatchItem *__restrict__ items,
+                                                  int nitems, BatchVecs v, int ci, int last,
+                                                  double *__restrict__ partials) {
+    batch_phase<OP>(d, items, nitems, v, ci, last, partials, blockIdx.x, gridDim.x);
+}
+
 __global__ __launch_bounds__(256) void k_batch_finalize(int op, SolverState *__restrict__ st,
                                                         const double *__restrict__ partials,
                                                         const int *__restrict__ item_start,
@@ -164,7 +174,195 @@ __global__ __launch_bounds__(256) void k_batch_all_done(const SolverState *__res
 
 __global__ void k_stamp(unsigned long long *t) { *t = wall_clock64(); }
 
+// Grid-wide barrier of the single-launch solve, with a bounded wait.  Barrier number g
+// (0, 1, 2, ... in every workgroup) is complete when the arrival counter reaches
+// (g + 1) * nwg: the counter is never reset, so no waiter can miss a release.  A
+// workgroup that waits longer than `limit` wall-clock ticks, or sees that another one gave
+// up, sets the error flag and returns false; its caller then ends the kernel, and so every
+// workgroup ends within one more limit.
+__device__ bool grid_wait(BatchBarrier *gb, unsigned long long target,
+                          unsigned long long limit) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence(); // this workgroup's writes before the barrier, visible device-wide
+        atomicAdd(&gb->arrived, 1ull);
+        const unsigned long long t0 = wall_clock64();
+        int ok = 0;
+        while (true) {
+            if (__hip_atomic_load(&gb->arrived, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >=
+                target) {
+                ok = 1;
+                break;
+            }
+            if (__hip_atomic_load(&gb->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                break;
+            if (wall_clock64() - t0 > limit) {
+                atomicExch(&gb->err, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence(); // later loads see the other workgroups' writes
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// The whole batched solve in one cooperative launch: the launch sequence of one iteration
+// (preconditioner, SpMVs with dot partials, per-system finalize, vector updates) runs as
+// phases of this kernel separated by grid-wide barriers, and the iterations loop on the
+// device until every system is done or max_iters is reached, so the host issues one
+// launch per solve and never polls.  Each phase is the same device code over the same
+// work items and partial slots as the separate launches, so the results are identical
+// to them.  Every workgroup takes every barrier: the phase sequence and the exit test
+// (all `done` flags, read after a barrier) are the same for all of them, and the loop is
+// bounded by max_iters; a barrier that times out ends the kernel (P.bar->err).
+__global__ __launch_bounds__(kBlock) void k_batch_solve(BatchSolve P) {
+    const int wg = blockIdx.x, nwg = gridDim.x;
+    const BatchDev &d = P.d;
+    unsigned long long gen = 0;
+    bool ok = true;
+    auto sync = [&]() __attribute__((always_inline)) {
+        if (ok)
+            ok = grid_wait(P.bar, (++gen) * (unsigned long long)nwg, P.wait_ticks);
+        return ok;
+    };
+    auto fin = [&](FinOp op) __attribute__((always_inline)) {
+        for (int s = wg; s < P.nsys; s += nwg) {
+            finalize_body(op, d.st + s, P.partials + (size_t)P.item_start[s] * kBatchPart,
+                          P.item_count[s], P.nvs[s], kBatchPart);
+            __syncthreads();
+        }
+        return sync();
+    };
+    // x = M^-1 b from a zero x (batch_precond)
+    auto precond = [&](const double *b, double *x) __attribute__((always_inline)) -> bool {
+        if (P.order > 0) {
+            const int k = P.order;
+            BatchVecs v;
+            v.b = b;
+            v.out = k == 1 ? x : P.cu1;
+            v.out2 = x;
+            v.out3 = P.cr;
+            batch_phase<B_CHEB0Z>(d, P.items, P.nitems, v, k - 1, k == 1, nullptr, wg, nwg);
+            if (!sync())
+                return false;
+            double *uc = P.cu1, *un = P.cu2;
+            for (int i = k - 2; i >= 0; --i) {
+                BatchVecs w;
+                w.in = uc;
+                w.o1 = P.cr;
+                w.out = un;
+                w.out2 = x;
+                batch_phase<B_CHEB>(d, P.items, P.nitems, w, i, i == 0, nullptr, wg, nwg);
+                if (!sync())
+                    return false;
+                double *t = uc;
+                uc = un;
+                un = t;
+            }
+            return true;
+        }
+        BatchVecs v;
+        v.b = b;
+        v.out = x;
+        batch_phase<B_COPY>(d, P.items, P.nitems, v, 0, 0, nullptr, wg, nwg);
+        return sync();
+    };
+    // r0 = b - A x ; p = r = r0 ; rho0
+    {
+        BatchVecs v;
+        v.in = P.x;
+        v.b = P.b;
+        v.out = P.r0;
+        v.out2 = P.p;
+        v.out3 = P.r;
+        batch_phase<B_RESID0>(d, P.items, P.nitems, v, 0, 0, P.partials, wg, nwg);
+        if (!sync() || !fin(FIN_INIT))
+            return;
+    }
+    __shared__ int s_all;
+    double *pp = P.pre ? P.p_ : P.p, *ss = P.pre ? P.s_ : P.s;
+    for (int it = 0; it <= P.max_iters; ++it) {
+        if (threadIdx.x == 0) {
+            int all = 1;
+            for (int s = 0; s < P.nsys; ++s)
+                all = all && d.st[s].done;
+            s_all = all;
+        }
+        __syncthreads();
+        if (s_all)
+            break; // the same decision in every workgroup: the flags were read after a barrier
+        __syncthreads();
+        if (P.pre && !precond(P.p, P.p_))
+            return;
+        BatchVecs v1;
+        v1.in = pp;
+        v1.o1 = P.r0;
+        v1.out = P.v;
+        batch_phase<B_DOT1>(d, P.items, P.nitems, v1, 0, 0, P.partials, wg, nwg);
+        if (!sync() || !fin(FIN_ALPHA))
+            return;
+        BatchVecs vs;
+        vs.b = P.r;
+        vs.o1 = P.v;
+        vs.out = P.s;
+        batch_phase<B_S_UPDATE>(d, P.items, P.nitems, vs, 0, 0, nullptr, wg, nwg);
+        if (!sync())
+            return;
+        if (P.pre && !precond(P.s, P.s_))
+            return;
+        BatchVecs v3;
+        v3.in = ss;
+        v3.o1 = P.s;
+        v3.out = P.t;
+        batch_phase<B_DOT3>(d, P.items, P.nitems, v3, 0, 0, P.partials, wg, nwg);
+        if (!sync() || !fin(FIN_OMEGA))
+            return;
+        BatchVecs vx;
+        vx.x = P.x;
+        vx.p_ = pp;
+        vx.s_ = ss;
+        vx.s = P.s;
+        vx.t = P.t;
+        vx.o1 = P.r0;
+        vx.out = P.r;
+        batch_phase<B_XR_UPDATE>(d, P.items, P.nitems, vx, 0, 0, P.partials, wg, nwg);
+        if (!sync() || !fin(FIN_BETA))
+            return;
+        BatchVecs vp;
+        vp.b = P.r;
+        vp.o1 = P.v;
+        vp.out = P.p;
+        batch_phase<B_P_UPDATE>(d, P.items, P.nitems, vp, 0, 0, nullptr, wg, nwg);
+        if (!sync())
+            return;
+    }
+}
+
 } // namespace
+
+int batch_solve_blocks() {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve, kBlock, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return 0;
+    per_cu = std::min(per_cu, 2); // well inside residency: 2 workgroups (8 waves) per CU
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+
+hipError_t batch_solve_launch(const BatchSolve &P, int grid, hipStream_t s) {
+    void *args[] = {(void *)&P};
+    return hipLaunchCooperativeKernel((const void *)k_batch_solve, dim3(grid), dim3(kBlock),
+                                      args, 0, s);
+}
 
 void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
                   const BatchVecs &v, int ci, int last, double *partials, hipStream_t s) {

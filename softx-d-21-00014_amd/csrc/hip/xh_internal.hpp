@@ -13,7 +13,7 @@ namespace xh {
 
 constexpr int kBlock = 256;
 constexpr int kMaxNV = 64;
-constexpr int kMaxPartials = 1024; // workgroups that write dot partials
+constexpr int kMaxPartials = 4096; // workgroups that write dot partials, at most
 
 enum Prec : int { F64 = 0, F32 = 1 };
 // u16: panel-relative column (col = base + u16); u8: compact column of a compressed block
@@ -154,6 +154,8 @@ int64_t csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32
 struct Tuning {
     int64_t strip_budget = 8 << 20; // bytes of x+y+A touched between reuses of an x row
     int tile_passes = 8;            // tile rows = tile_passes * rows per workgroup pass
+    int min_passes = 1;             // row-order schedule: passes per workgroup at least
+    int dot_groups = 1024;          // workgroups of a launch that writes dot partials
     int fold_enable = 1;            // 0: always contiguous order
     int64_t fold_min_bytes = 2 << 20; // fold only when 2*fold*nv*sizeof(value) exceeds this
     // 1: structured-grid matrices run the LDS-window line sweep (k_spmv_grid).  Off by
@@ -368,6 +370,32 @@ enum BatchOp : int {
     B_XR_UPDATE = 7, // x = ac p_ + oc s_ + x ; out = s - omega t ; (out, o1)
     B_P_UPDATE = 8,  // out = b + pcoef o1 + beta out
 };
+
+// Operands of the single-launch batched solve (k_batch_solve): the systems, their work
+// items and item ranges, the solver vectors, and the preconditioner (Chebyshev order, or 0
+// for a copy).
+// arrival counter and error flag of the single-launch solve's grid barriers (zeroed
+// before each launch)
+struct BatchBarrier {
+    unsigned long long arrived;
+    int err;
+};
+struct BatchSolve {
+    BatchDev d;
+    BatchBarrier *bar = nullptr;
+    unsigned long long wait_ticks = 0; // longest barrier wait, wall-clock ticks
+    const BatchItem *items = nullptr;
+    int nitems = 0, nsys = 0, max_iters = 0, pre = 0, order = 0;
+    const int *item_start = nullptr, *item_count = nullptr, *nvs = nullptr;
+    double *partials = nullptr;
+    double *x = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr, *s = nullptr;
+    double *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cr = nullptr, *cu1 = nullptr,
+           *cu2 = nullptr;
+    const double *b = nullptr;
+};
+// workgroups of k_batch_solve that are resident at once (0 when the query fails)
+int batch_solve_blocks();
+hipError_t batch_solve_launch(const BatchSolve &P, int grid, hipStream_t s);
 
 void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
                   const BatchVecs &v, int ci, int last, double *partials, hipStream_t s);

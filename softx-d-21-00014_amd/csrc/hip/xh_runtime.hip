@@ -1878,6 +1878,10 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.strip_budget = value;
     else if (k == "spmv_tile_passes" && value >= 1 && value <= 1024)
         g_tune.tile_passes = (int)value;
+    else if (k == "spmv_min_passes" && value >= 1 && value <= 1024)
+        g_tune.min_passes = (int)value;
+    else if (k == "dot_groups" && value >= 8 && value <= kMaxPartials && value % 8 == 0)
+        g_tune.dot_groups = (int)value;
     else if (k == "spmv_fold_enable" && (value == 0 || value == 1))
         g_tune.fold_enable = (int)value;
     else if (k == "spmv_fold_min_bytes" && value >= 0)
@@ -1920,7 +1924,7 @@ extern "C" int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int pr
     const int vb = prec == F64 ? 8 : 4;
     const int w = std::min(16 / vb, nv);
     const int rpb = kBlock / (nv / w);
-    const Launch L = spmv_plan(nrows, nnz, fold, rpb, dots ? kMaxPartials : 16384,
+    const Launch L = spmv_plan(nrows, nnz, fold, rpb, dots ? g_tune.dot_groups : 16384,
                                (int64_t)nv * vb);
     if (grid)
         *grid = L.grid;
@@ -1961,7 +1965,7 @@ extern "C" int xh_grid_schedule(int32_t nx, int32_t ny, int32_t nz, int prec, in
     const int vb = prec == F64 ? 8 : 4;
     const int w = std::min(16 / vb, nv);
     const int rpb = kBlock / (nv / w);
-    const GridLaunch G = grid_plan(nx, ny, nz, rpb, dots ? kMaxPartials : 16384);
+    const GridLaunch G = grid_plan(nx, ny, nz, rpb, dots ? g_tune.dot_groups : 16384);
     if (grid)
         *grid = G.grid;
     if (nitems)
@@ -2169,11 +2173,13 @@ struct xh_batch_s {
     double *s = nullptr, *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cr = nullptr,
            *cu1 = nullptr, *cu2 = nullptr;
     unsigned long long *d_t0 = nullptr;
+    BatchBarrier *d_bar = nullptr; // grid barriers of the single-launch solve
     // host
     std::vector<BatchItem> items;
     std::vector<int> item_start, item_count;
     int tile_passes[5] = {8, 8, 8, 8, 8}; // per nv class 1, 2, 4, 8, 16
     int grid = 2048;
+    int persistent = 0; // 1: the whole solve is one cooperative launch (k_batch_solve)
     SolverState *hst = nullptr; // pinned [nsys]
     int *hflag = nullptr;       // pinned [2][2]
     unsigned long long *ht0 = nullptr;
@@ -2344,7 +2350,7 @@ void free_batch(xh_batch_s *B) {
                     (void *)B->d_items, (void *)B->st, (void *)B->partials, (void *)B->x,
                     (void *)B->b, (void *)B->r0, (void *)B->r, (void *)B->p, (void *)B->v,
                     (void *)B->s, (void *)B->t, (void *)B->p_, (void *)B->s_, (void *)B->cr,
-                    (void *)B->cu1, (void *)B->cu2, (void *)B->d_t0})
+                    (void *)B->cu1, (void *)B->cu2, (void *)B->d_t0, (void *)B->d_bar})
         if (q)
             (void)hipFree(q);
     if (B->hst)
@@ -2456,7 +2462,8 @@ extern "C" int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nv
         (rc = dmalloc(&B->d_rbase, nsys, acc)) || (rc = dmalloc(&B->d_voff, nsys, acc)) ||
         (rc = dmalloc(&B->d_nv, nsys, acc)) || (rc = dmalloc(&B->d_item_start, nsys, acc)) ||
         (rc = dmalloc(&B->d_item_count, nsys, acc)) || (rc = dmalloc(&B->d_flag, 2, acc)) ||
-        (rc = dmalloc(&B->st, nsys, acc)) || (rc = dmalloc(&B->d_t0, 1, acc)))
+        (rc = dmalloc(&B->st, nsys, acc)) || (rc = dmalloc(&B->d_t0, 1, acc)) ||
+        (rc = dmalloc(&B->d_bar, 1, acc)))
         return bail(rc);
     for (double **q : {&B->x, &B->b, &B->r0, &B->r, &B->p, &B->v, &B->s, &B->t, &B->p_, &B->s_,
                        &B->cr, &B->cu1, &B->cu2})
@@ -2533,6 +2540,49 @@ extern "C" int xh_batch_solve(xh_batch B) {
     XH_HIP(hipMemcpyAsync(B->st, B->hst, sizeof(SolverState) * nsys, hipMemcpyHostToDevice, st));
     fill_zero(B->velems, F64, B->x, nullptr, st);
     stamp_clock(B->d_t0, st);
+    if (B->persistent) {
+        // init and every iteration in one cooperative launch; no host round trip
+        BatchSolve P;
+        P.d = batch_dev(B);
+        P.items = B->d_items;
+        P.nitems = (int)B->items.size();
+        P.nsys = nsys;
+        P.max_iters = std::max(1, B->prm.solver.max_iters);
+        P.pre = B->prm.solver.method != XH_BICGSTAB;
+        P.order = B->prm.precond.method == XH_CHEBYSHEV ? B->prm.precond.poly_order : 0;
+        P.item_start = B->d_item_start;
+        P.item_count = B->d_item_count;
+        P.nvs = B->d_nv;
+        P.partials = B->partials;
+        P.x = B->x;
+        P.b = B->b;
+        P.r0 = B->r0;
+        P.r = B->r;
+        P.p = B->p;
+        P.v = B->v;
+        P.s = B->s;
+        P.t = B->t;
+        P.p_ = B->p_;
+        P.s_ = B->s_;
+        P.cr = B->cr;
+        P.cu1 = B->cu1;
+        P.cu2 = B->cu2;
+        const int cap = batch_solve_blocks();
+        if (cap < 1)
+            return fail(XH_EHIP, "xh_batch_solve: no resident workgroup count for the "
+                                 "single-launch solve");
+        const int grid = std::max(1, std::min(cap, std::max(P.nitems, nsys)));
+        P.bar = B->d_bar;
+        P.wait_ticks = (unsigned long long)(B->wall_khz * 2000.0); // 2 s per barrier
+        XH_HIP(hipMemsetAsync(B->d_bar, 0, sizeof(BatchBarrier), st));
+        XH_HIP(batch_solve_launch(P, grid, st));
+        BatchBarrier hb;
+        XH_HIP(hipMemcpyAsync(&hb, B->d_bar, sizeof(BatchBarrier), hipMemcpyDeviceToHost, st));
+        XH_HIP(hipStreamSynchronize(st));
+        if (hb.err)
+            return fail(XH_EHIP, "xh_batch_solve: a grid barrier of the single-launch solve "
+                                 "timed out (the kernel ended; the solution is not valid)");
+    } else {
     // r0 = b - A x ; p = r = r0 ; rho0 (get_residual + copies, pbicgstab.inl:111-117)
     BatchVecs v;
     v.in = B->x;
@@ -2571,6 +2621,7 @@ extern "C" int xh_batch_solve(xh_batch B) {
             if (B->hflag[2 * ((it - 1) & 1)])
                 break;
         }
+    }
     }
     XH_HIP(hipMemcpyAsync(B->hst, B->st, sizeof(SolverState) * nsys, hipMemcpyDeviceToHost, st));
     XH_HIP(hipMemcpyAsync(B->ht0, B->d_t0, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -2680,6 +2731,13 @@ extern "C" int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes) {
     for (int c = 0; c < 5; ++c)
         B->tile_passes[c] = tile_passes[c];
     return upload_items(B);
+}
+
+extern "C" int xh_batch_set_persistent(xh_batch B, int on) {
+    if (!B || (on != 0 && on != 1))
+        return fail(XH_EINVAL, "xh_batch_set_persistent: bad arguments");
+    B->persistent = on;
+    return 0;
 }
 
 extern "C" int xh_batch_destroy(xh_batch B) {

@@ -215,11 +215,19 @@ __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
                                         uint32_t base, const VT *__restrict__ x, int k0,
                                         VT (&acc)[W]) {
     using C = Chunk<VT, NV, W>;
-    consume_chunk<VT, NV, W>(j, e, first, base, x, k0, acc);
-    for (j += C::CH; j < e; j += C::CH) {
-        C ch;
-        load_chunk<VT, IT, NV, W>(j, e, col, val, ch);
-        consume_chunk<VT, NV, W>(j, e, ch, base, x, k0, acc);
+    // a longer row loads its next chunk before the x loads of the current one, so the
+    // chunk's latency overlaps them
+    C cur = first;
+    while (true) {
+        const bool more = j + C::CH < e; // the same for all lanes of the row
+        C nxt;
+        if (more)
+            load_chunk<VT, IT, NV, W>(j + C::CH, e, col, val, nxt);
+        consume_chunk<VT, NV, W>(j, e, cur, base, x, k0, acc);
+        if (!more)
+            break;
+        j += C::CH;
+        cur = nxt;
     }
 }
 
@@ -1054,7 +1062,9 @@ Launch spmv_plan(int64_t n, int64_t nnz, int64_t F, int rpb, int cap, int64_t ro
         s.tps = (int32_t)((s.strip + t - 1) / t);
         s.ntiles = ((F + s.strip - 1) / s.strip) * s.na * s.tps;
     } else {
+        // at least min_passes passes per workgroup, so the next row's loads overlap
         int64_t need = (n + rpb - 1) / rpb;
+        need = std::max<int64_t>(1, need / std::max(1, g_tune.min_passes));
         int64_t g = std::min<int64_t>(need, cap);
         g = std::max<int64_t>(8, (g + 7) / 8 * 8);
         int64_t chunk = (n + g - 1) / g;
@@ -1141,7 +1151,7 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int ND = NDots<OP>::value;
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
-    const int cap = ND > 0 ? kMaxPartials : (a.grid_cap > 0 ? a.grid_cap : 16384);
+    const int cap = ND > 0 ? g_tune.dot_groups : (a.grid_cap > 0 ? a.grid_cap : 16384);
     if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 && NV >= 8) {
         if (A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr) {
             constexpr int G = 64 / (NV / W);

@@ -132,6 +132,7 @@ def _load():
         "xh_batch_stats": [vp, C.c_int, P(i32), vp, vp, vp, P(f64)],
         "xh_batch_autotune": [vp, C.c_int, vp],
         "xh_batch_set_tiles": [vp, vp],
+        "xh_batch_set_persistent": [vp, C.c_int],
         "xh_batch_destroy": [vp],
         "xh_unet_create": [C.c_int, vp, u64, P(vp)],
         "xh_unet_destroy": [vp],
@@ -666,6 +667,10 @@ class Batch:
         check(hip().xh_batch_stats(self.h, s, C.byref(it), _ptr(conv), _ptr(a), _ptr(r),
                                    C.byref(lat)), "xh_batch_stats")
         return dict(iters=it.value, conv=conv, abs_res=a, rel_res=r, latency_ms=lat.value)
+
+    def set_persistent(self, on=True):
+        """solve in one cooperative launch (True) or one launch sequence per iteration"""
+        check(hip().xh_batch_set_persistent(self.h, int(on)), "xh_batch_set_persistent")
 
     def autotune(self, reps=5):
         t = np.zeros(5, np.int32)

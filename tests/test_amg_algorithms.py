@@ -212,6 +212,27 @@ def test_constants_are_interpolated_exactly(X, itype, coarsen):
         assert np.all(D[cf == 1].sum(1) == 1.0)
 
 
+@pytest.mark.parametrize("itype", [0, 6])
+def test_strong_f_connections_are_distributed_exactly(X, itype):
+    # on the 2-D 9-point Laplacian every off-diagonal is strong and RS leaves F points
+    # with strong F neighbours; classical interpolation distributes each such a_ik over
+    # the common C points (extended+i over its wider set) in proportion to a_kj, so a
+    # zero-row-sum row still interpolates constants exactly -- a wrong share of a_ik
+    # breaks the row sum
+    nx = 14
+    row, col, val = lap2d(nx, nx, nine=True)
+    s = X.amg_strength(row, col, val, 0.25, 0.9)
+    cf = X.amg_split(row, col, val, s, coarsen_type=6)
+    n = nx * nx
+    ff = [i for i in range(n) if cf[i] == -1 and any(cf[j] == -1 for j in neighbours(row, col, i))]
+    assert len(ff) > n // 4
+    D = dense_p(X.amg_interp(row, col, val, s, cf, interp_type=itype))
+    rs = np.add.reduceat(val, row[:-1].astype(np.int64))
+    interior = [i for i in ff if abs(rs[i]) < 1e-12]
+    assert len(interior) > 0
+    assert np.abs(D[interior].sum(1) - 1.0).max() < 1e-13
+
+
 def test_truncation_keeps_row_sums_and_drops_small_weights(X):
     N = 8
     row, col, val, _ = X.varcoef7(N, sigma=1.5, seed=9)

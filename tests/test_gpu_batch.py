@@ -91,6 +91,35 @@ def test_batch_graph_eager_and_retiling(gpu):
     assert all(B.stats(s)["conv"].all() for s in range(len(sizes)))
 
 
+@pytest.mark.parametrize("method,precond", [("PBiCGStab", "Chebyshev"), ("BiCGStab", None),
+                                            ("MergedPBiCGStab", "Identity")])
+def test_batch_single_launch_is_identical(gpu, O, method, precond):
+    # the cooperative single-launch solve runs the same phases over the same work items
+    # and partial slots as the per-iteration launches: identical iterates; a second solve
+    # on the same batch gives them again
+    X = gpu
+    sizes = (6, 9, 12, 7, 10)
+    nvs = (4, 1, 16, 8, 2)
+    sy = systems(X, sizes, seed=40, sigma=1.0 if precond == "Chebyshev" else 0.0)
+    prm = params(X, method, precond)
+    B, bs = run_batch(X, sy, nvs, prm)
+    ref = [(B.x(s), B.stats(s)) for s in range(len(sizes))]
+    B.set_persistent(True)
+    for _ in range(2):
+        B.solve()
+        for s in range(len(sizes)):
+            st = B.stats(s)
+            assert st["iters"] == ref[s][1]["iters"]
+            assert np.array_equal(st["conv"], ref[s][1]["conv"])
+            assert np.array_equal(B.x(s), ref[s][0]), (method, s)
+            assert st["latency_ms"] > 0
+    # and against the oracle's single-system solve
+    for s, (r, c, v, lv) in enumerate(sy):
+        o = O.solve(O.Hier(lv), O.params_from(prm), nvs[s], bs[s])
+        assert o["conv"].all() and abs(B.stats(s)["iters"] - o["iters"]) <= 1
+        assert rel_diff(B.x(s), o["x"], nvs[s]) <= 1e-6
+
+
 def test_batch_rejects_unsupported(gpu):
     X = gpu
     sy = systems(X, (5,))

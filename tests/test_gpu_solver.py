@@ -158,6 +158,25 @@ def test_preconditioner_bit_exact(gpu, O, smoother, k, mg_prec):
     assert np.array_equal(dx.to_host(), O.mg_apply(O.Hier(levels), O.params_from(prm), nv, b))
 
 
+@pytest.mark.parametrize("mg_prec", [0, 2])
+def test_hsgs_core_path_bit_exact(gpu, O, mg_prec):
+    # blocks small enough for the staged core-block sweep in both precisions (32-row
+    # blocks: split_range gives 40 rows); the launches of a solve show that kernel ran
+    X = gpu
+    N, nv = 20, 8
+    levels = levels_for(X, N)
+    prm = X.default_params(method="MergedPBiCGStab", smoother="HSGS", sweeps=1, mg_prec=mg_prec)
+    prm.hsgs_block_rows = 32
+    S = X.Solver(X.Hier(levels), prm, nv)
+    b = X.rhs_uniform(9, N ** 3, nv)
+    db, dx = X.DeviceArray.from_host(b), X.DeviceArray(N ** 3 * nv)
+    S.precond_apply(db, dx)
+    assert np.array_equal(dx.to_host(), O.mg_apply(O.Hier(levels), O.params_from(prm), nv, b))
+    dx.zero()
+    names = {k["kernel"] for k in S.profile(db, dx, 1)}
+    assert f"k_hsgs_core<{'double' if mg_prec == 0 else 'float'},{nv}>" in names, names
+
+
 def test_graph_and_eager_are_identical(gpu):
     X = gpu
     N, nv = 24, 8
