@@ -390,7 +390,8 @@ def main():
             "workload": "64 independent 7-point variable-coefficient systems (k lognormal, "
                         "sigma 1), edge uniform in {20..64}, nv uniform in {1,2,4,8,16}, "
                         "BiCGStab + degree-8 Chebyshev (Jacobi-scaled), rel-res 1e-8, one "
-                        "launch sequence per iteration for all systems (hipGraph)",
+                        "launch sequence per iteration for all systems (hipGraph); work-item "
+                        "rows and RHS per lane from the autotuner",
             "rows_total": int(sum(int(q) ** 3 for q in sizes)),
             "rhs_total": int(nvs.sum()),
             "ms_per_batch": round(ms_b, 3),
@@ -401,6 +402,7 @@ def main():
             "iters_min_max": [min(q["iters"] for q in st_), max(q["iters"] for q in st_)],
             "all_converged": bool(all(q["conv"].all() for q in st_)),
             "tile_passes_nv_1_2_4_8_16": [int(t) for t in tiles],
+            "rhs_per_lane_nv_1_2_4_8_16": Bt.layout()[1],
             "single_launch": single,
         }
         del Bt

@@ -328,10 +328,16 @@ int xh_batch_get_x(xh_batch B, int s, double *x);
 int xh_batch_stats(xh_batch B, int s, int32_t *iters, int32_t *conv, double *abs_res,
                    double *rel_res, double *latency_ms);
 /* Autotuner: for each nv class present (1, 2, 4, 8, 16) times `reps` batched SpMVs per
- * candidate work-item size (1..32 passes of 256/nv rows) and keeps the fastest;
- * tile_passes (nullable) receives the 5 choices.  xh_batch_set_tiles sets them. */
+ * candidate layout -- RHS per lane (the nv tile width of a lane: 1, or 2 for nv >= 2, so
+ * nv or nv/2 lanes per row) times work-item size (1..32 passes of 256*w/nv rows) -- and
+ * keeps the fastest; tile_passes (nullable) receives the 5 pass choices.
+ * xh_batch_set_tiles sets the passes, xh_batch_set_layout passes and RHS per lane,
+ * xh_batch_get_layout reads both.  The layout changes only the grouping of the dot
+ * partial sums, not any RHS's arithmetic. */
 int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes);
 int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes);
+int xh_batch_set_layout(xh_batch B, const int32_t *tile_passes, const int32_t *rhs_per_lane);
+int xh_batch_get_layout(xh_batch B, int32_t *tile_passes, int32_t *rhs_per_lane);
 /* on = 1: every later xh_batch_solve is one cooperative (persistent) launch: the init step
  * and the iterations run on the device, the launch sequence of an iteration as phases
  * separated by grid-wide barriers, until every system is done or max_iters; the host

@@ -18,16 +18,21 @@
 namespace xh {
 namespace {
 
-// per-item reduction of nd dots over the rows of the item, fixed order
-__device__ __forceinline__ void item_partials(const double (&part)[3], int nd, int nv,
+// per-item reduction of nd dots over the rows of the item, fixed order: lane tid holds W
+// consecutive RHS of one row, so red[d][tid*W + w] is row tid / (nv/W), RHS
+// (tid % (nv/W))*W + w, i.e. red[d][q*nv + k] for pass row q and RHS k
+template <int W>
+__device__ __forceinline__ void item_partials(const double (&part)[3][W], int nd, int nv,
                                               double *__restrict__ out) {
-    __shared__ double red[3][kBlock];
+    __shared__ double red[3][kBlock * 2];
     const int tid = threadIdx.x;
     for (int d = 0; d < nd; ++d)
-        red[d][tid] = part[d];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            red[d][tid * W + w] = part[d][w];
     __syncthreads();
     if (tid < nd * nv) {
-        const int d = tid / nv, k = tid % nv, rpp = kBlock / nv;
+        const int d = tid / nv, k = tid % nv, rpp = kBlock * W / nv;
         double s = red[d][k];
         for (int q = 1; q < rpp; ++q)
             s = s + red[d][q * nv + k];
@@ -36,104 +41,145 @@ __device__ __forceinline__ void item_partials(const double (&part)[3], int nd, i
     __syncthreads();
 }
 
-// one batched operation over the work items wg, wg + nwg, ... (one item per workgroup at
-// a time)
-template <int OP>
-__device__ __forceinline__ void batch_phase(const BatchDev &d,
-                                            const BatchItem *__restrict__ items, int nitems,
-                                            const BatchVecs &v, int ci, int last,
-                                            double *__restrict__ partials, int wg, int nwg) {
+// One work item of one batched operation with W right-hand sides per lane (nv/W lanes
+// per row, kBlock*W/nv rows per pass).  Each RHS gets the arithmetic of the single-system
+// kernels; W changes only which lane holds it and so the grouping of the dot partials.
+template <int OP, int W>
+__device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w,
+                                           const BatchVecs &v, int ci, int last,
+                                           double *__restrict__ partials, int it) {
     constexpr int ND = (OP == B_RESID0 || OP == B_DOT1 || OP == B_XR_UPDATE) ? 1
                        : (OP == B_DOT3)                                      ? 3
                                                                              : 0;
     const int tid = threadIdx.x;
-    for (int it = wg; it < nitems; it += nwg) {
-        const BatchItem w = items[it];
-        const SolverState *st = d.st + w.sys;
-        if (st->done)
-            continue; // uniform over the workgroup: one item per workgroup at a time
-        const int nv = w.nv, rpp = kBlock / nv;
-        const int tr = tid / nv, k = tid % nv;
-        const int64_t gb = d.rbase[w.sys];
-        const size_t vo = (size_t)d.voff[w.sys];
-        double part[3] = {0.0, 0.0, 0.0};
-        double c1 = 0, c2 = 0, c3 = 0;
+    const SolverState *st = d.st + w.sys;
+    const int nv = w.nv, lanes = nv / W, rpp = kBlock / lanes;
+    const int tr = tid / lanes, k0 = (tid % lanes) * W;
+    const int64_t gb = d.rbase[w.sys];
+    const size_t vo = (size_t)d.voff[w.sys];
+    double part[3][W];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int u = 0; u < W; ++u)
+            part[q][u] = 0.0;
+    double c1[W], c2[W], c3[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+        const int k = k0 + u;
+        c1[u] = c2[u] = c3[u] = 0.0;
         if constexpr (OP == B_S_UPDATE) {
-            c1 = st->nalpha[k];
+            c1[u] = st->nalpha[k];
         } else if constexpr (OP == B_XR_UPDATE) {
-            c1 = st->ac[k];
-            c2 = st->oc[k];
-            c3 = st->omega[k];
+            c1[u] = st->ac[k];
+            c2[u] = st->oc[k];
+            c3[u] = st->omega[k];
         } else if constexpr (OP == B_P_UPDATE) {
-            c1 = st->pcoef[k];
-            c2 = st->beta[k];
+            c1[u] = st->pcoef[k];
+            c2[u] = st->beta[k];
         } else if constexpr (OP == B_CHEB0Z || OP == B_CHEB) {
-            c1 = d.cheb[(size_t)w.sys * kChebMax + ci];
+            c1[u] = d.cheb[(size_t)w.sys * kChebMax + ci];
         }
-        if (tr < rpp)
-            for (int64_t i = w.r0 + tr; i < w.r1; i += rpp) {
-                const size_t o = vo + (size_t)i * nv + k;
-                const int64_t g = gb + i;
-                if constexpr (OP == B_RESID0 || OP == B_DOT1 || OP == B_DOT3 || OP == B_CHEB) {
-                    const double *xs = v.in + vo;
-                    double acc = 0.0;
-                    for (uint32_t j = d.row[g]; j < d.row[g + 1]; ++j)
-                        acc = acc + xs[(size_t)d.col[j] * nv + k] * d.val[j];
+    }
+    if (tr < rpp)
+        for (int64_t i = w.r0 + tr; i < w.r1; i += rpp) {
+            const size_t ob = vo + (size_t)i * nv + k0;
+            const int64_t g = gb + i;
+            if constexpr (OP == B_RESID0 || OP == B_DOT1 || OP == B_DOT3 || OP == B_CHEB) {
+                const double *xs = v.in + vo;
+                double acc[W];
+#pragma unroll
+                for (int u = 0; u < W; ++u)
+                    acc[u] = 0.0;
+                for (uint32_t j = d.row[g]; j < d.row[g + 1]; ++j) {
+                    const size_t xo = (size_t)d.col[j] * nv + k0;
+                    const double a = d.val[j];
+#pragma unroll
+                    for (int u = 0; u < W; ++u)
+                        acc[u] = acc[u] + xs[xo + u] * a;
+                }
+#pragma unroll
+                for (int u = 0; u < W; ++u) {
+                    const size_t o = ob + u;
                     if constexpr (OP == B_RESID0) {
                         // r0 = b - A x ; p = r = r0 ; (r0, r0)
-                        const double t = v.b[o] - acc;
-                        part[0] = part[0] + t * t;
+                        const double t = v.b[o] - acc[u];
+                        part[0][u] = part[0][u] + t * t;
                         v.out[o] = t;
                         v.out2[o] = t;
                         v.out3[o] = t;
                     } else if constexpr (OP == B_DOT1) {
-                        v.out[o] = acc;
-                        part[0] = part[0] + acc * v.o1[o];
+                        v.out[o] = acc[u];
+                        part[0][u] = part[0][u] + acc[u] * v.o1[o];
                     } else if constexpr (OP == B_DOT3) {
                         const double sv = v.o1[o];
-                        v.out[o] = acc;
-                        part[0] = part[0] + acc * sv;
-                        part[1] = part[1] + acc * acc;
-                        part[2] = part[2] + sv * sv;
+                        v.out[o] = acc[u];
+                        part[0][u] = part[0][u] + acc[u] * sv;
+                        part[1][u] = part[1][u] + acc[u] * acc[u];
+                        part[2][u] = part[2][u] + sv * sv;
                     } else { // B_CHEB: t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
                         const double di = d.isd[g];
-                        const double t2 = acc * di;
-                        double t1 = c1 * v.o1[o] + t2;
+                        const double t2 = acc[u] * di;
+                        double t1 = c1[u] * v.o1[o] + t2;
                         t1 = t1 * di;
                         if (last)
                             v.out2[o] = t1; // x is zero-flagged: x = t1
                         else
                             v.out[o] = t1;
                     }
-                } else if constexpr (OP == B_CHEB0Z) {
-                    // residual of a zero-flagged x is b: r = b*isd ; t1 = (c*r)*isd
-                    const double di = d.isd[g];
-                    const double rv = v.b[o] * di;
-                    double t1 = c1 * rv;
-                    t1 = t1 * di;
-                    if (last) {
-                        v.out2[o] = t1;
-                    } else {
-                        v.out3[o] = rv;
-                        v.out[o] = t1;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < W; ++u) {
+                    const size_t o = ob + u;
+                    if constexpr (OP == B_CHEB0Z) {
+                        // residual of a zero-flagged x is b: r = b*isd ; t1 = (c*r)*isd
+                        const double di = d.isd[g];
+                        const double rv = v.b[o] * di;
+                        double t1 = c1[u] * rv;
+                        t1 = t1 * di;
+                        if (last) {
+                            v.out2[o] = t1;
+                        } else {
+                            v.out3[o] = rv;
+                            v.out[o] = t1;
+                        }
+                    } else if constexpr (OP == B_COPY) {
+                        v.out[o] = v.b[o];
+                    } else if constexpr (OP == B_S_UPDATE) {
+                        v.out[o] = v.b[o] + c1[u] * v.o1[o]; // s = r + (-alpha) v
+                    } else if constexpr (OP == B_XR_UPDATE) {
+                        // x = ac p_ + oc s_ + x ; r = s - omega t ; (r, r0)
+                        v.x[o] = c1[u] * v.p_[o] + c2[u] * v.s_[o] + v.x[o];
+                        const double rv = v.s[o] - c3[u] * v.t[o];
+                        v.out[o] = rv;
+                        part[0][u] = part[0][u] + rv * v.o1[o];
+                    } else if constexpr (OP == B_P_UPDATE) {
+                        // p = r + (-(omega beta)) v + beta p
+                        v.out[o] = v.b[o] + c1[u] * v.o1[o] + c2[u] * v.out[o];
                     }
-                } else if constexpr (OP == B_COPY) {
-                    v.out[o] = v.b[o];
-                } else if constexpr (OP == B_S_UPDATE) {
-                    v.out[o] = v.b[o] + c1 * v.o1[o]; // s = r + (-alpha) v
-                } else if constexpr (OP == B_XR_UPDATE) {
-                    // x = ac p_ + oc s_ + x ; r = s - omega t ; (r, r0)
-                    v.x[o] = c1 * v.p_[o] + c2 * v.s_[o] + v.x[o];
-                    const double rv = v.s[o] - c3 * v.t[o];
-                    v.out[o] = rv;
-                    part[0] = part[0] + rv * v.o1[o];
-                } else if constexpr (OP == B_P_UPDATE) {
-                    // p = r + (-(omega beta)) v + beta p
-                    v.out[o] = v.b[o] + c1 * v.o1[o] + c2 * v.out[o];
                 }
             }
-        if constexpr (ND > 0)
-            item_partials(part, ND, nv, partials + (size_t)it * kBatchPart);
+        }
+    if constexpr (ND > 0)
+        item_partials<W>(part, ND, nv, partials + (size_t)it * kBatchPart);
+}
+
+// one batched operation over the work items wg, wg + nwg, ... (one item per workgroup at
+// a time); an item's `wpl` is its RHS per lane
+template <int OP>
+__device__ __forceinline__ void batch_phase(const BatchDev &d,
+                                            const BatchItem *__restrict__ items, int nitems,
+                                            const BatchVecs &v, int ci, int last,
+                                            double *__restrict__ partials, int wg, int nwg) {
+    for (int it = wg; it < nitems; it += nwg) {
+        const BatchItem w = items[it];
+        if (d.st[w.sys].done)
+            continue; // uniform over the workgroup: one item per workgroup at a time
+        if (w.wpl == 2)
+            batch_item<OP, 2>(d, w, v, ci, last, partials, it);
+        else
+            batch_item<OP, 1>(d, w, v, ci, last, partials, it);
     }
 }
 

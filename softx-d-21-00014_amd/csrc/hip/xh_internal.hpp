@@ -334,6 +334,7 @@ constexpr int kBatchMaxNV = 16;
 struct BatchItem {        // rows [r0, r1) of system `sys`, which has nv RHS
     int32_t sys, nv;
     int64_t r0, r1;
+    int32_t wpl, pad;      // RHS per lane (1 or 2: nv/wpl lanes per row)
 };
 
 struct BatchDev {

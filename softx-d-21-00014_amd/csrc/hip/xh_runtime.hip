@@ -2178,6 +2178,7 @@ struct xh_batch_s {
     std::vector<BatchItem> items;
     std::vector<int> item_start, item_count;
     int tile_passes[5] = {8, 8, 8, 8, 8}; // per nv class 1, 2, 4, 8, 16
+    int tile_w[5] = {1, 1, 1, 1, 1};      // RHS per lane per nv class (1 or 2)
     int grid = 2048;
     int persistent = 0; // 1: the whole solve is one cooperative launch (k_batch_solve)
     SolverState *hst = nullptr; // pinned [nsys]
@@ -2208,11 +2209,12 @@ void build_items(xh_batch_s *B, unsigned mask, std::vector<BatchItem> &items,
         const int c = nv_class(B->nv[s]);
         if (!(mask & (1u << c)))
             continue;
-        const int64_t tile = (int64_t)B->tile_passes[c] * (kBlock / B->nv[s]);
+        const int wpl = B->nv[s] >= 2 ? B->tile_w[c] : 1;
+        const int64_t tile = (int64_t)B->tile_passes[c] * (kBlock * wpl / B->nv[s]);
         if (start)
             (*start)[s] = (int)items.size();
         for (int64_t r0 = 0; r0 < B->n[s]; r0 += tile)
-            items.push_back({s, B->nv[s], r0, std::min(B->n[s], r0 + tile)});
+            items.push_back({s, B->nv[s], r0, std::min(B->n[s], r0 + tile), wpl, 0});
         if (count)
             (*count)[s] = (int)items.size() - (*start)[s];
     }
@@ -2678,9 +2680,11 @@ extern "C" int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes) {
         if (!present)
             continue;
         double best = 1e30;
-        int best_p = B->tile_passes[c];
-        for (int p : cand) {
+        int best_p = B->tile_passes[c], best_w = B->tile_w[c];
+        for (int cw = 0; cw < (c == 0 ? 6 : 12); ++cw) {
+            const int p = cand[cw % 6], wl = 1 + cw / 6;
             B->tile_passes[c] = p;
+            B->tile_w[c] = wl;
             build_items(B, 1u << c, items, nullptr, nullptr);
             (void)hipFree(d_items);
             (void)hipFree(d_part);
@@ -2706,9 +2710,11 @@ extern "C" int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes) {
             if (ms < best) {
                 best = ms;
                 best_p = p;
+                best_w = wl;
             }
         }
         B->tile_passes[c] = best_p;
+        B->tile_w[c] = best_w;
     }
     (void)hipFree(d_items);
     (void)hipFree(d_part);
@@ -2730,6 +2736,34 @@ extern "C" int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes) {
             return fail(XH_EINVAL, "xh_batch_set_tiles: tile passes must be 1..1024");
     for (int c = 0; c < 5; ++c)
         B->tile_passes[c] = tile_passes[c];
+    return upload_items(B);
+}
+
+extern "C" int xh_batch_get_layout(xh_batch B, int32_t *tile_passes, int32_t *rhs_per_lane) {
+    if (!B)
+        return fail(XH_EINVAL, "xh_batch_get_layout: null batch");
+    for (int c = 0; c < 5; ++c) {
+        if (tile_passes)
+            tile_passes[c] = B->tile_passes[c];
+        if (rhs_per_lane)
+            rhs_per_lane[c] = c == 0 ? 1 : B->tile_w[c];
+    }
+    return 0;
+}
+
+extern "C" int xh_batch_set_layout(xh_batch B, const int32_t *tile_passes,
+                                   const int32_t *rhs_per_lane) {
+    if (!B || !tile_passes || !rhs_per_lane)
+        return fail(XH_EINVAL, "xh_batch_set_layout: bad arguments");
+    for (int c = 0; c < 5; ++c)
+        if (tile_passes[c] < 1 || tile_passes[c] > 1024 ||
+            (rhs_per_lane[c] != 1 && rhs_per_lane[c] != 2) || (c == 0 && rhs_per_lane[c] != 1))
+            return fail(XH_EINVAL, "xh_batch_set_layout: passes 1..1024, 1 or 2 RHS per lane "
+                                   "(1 for nv = 1)");
+    for (int c = 0; c < 5; ++c) {
+        B->tile_passes[c] = tile_passes[c];
+        B->tile_w[c] = rhs_per_lane[c];
+    }
     return upload_items(B);
 }
 

@@ -926,25 +926,30 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st7(
             return ld<VT, W>(x + (size_t)r * NV + k0);
         };
         // registers of step cc: border neighbours (y: S/N, z: D/U), values, epilogue
+        // With two or more lines per direction a row loads one border line in y (south
+        // for the first line of the block, north for the last; none in between) and one
+        // in z, held in one register set each.
         struct Step {
-            P bs, bn, bd, bu;
+            P by0, by1, bz0, bz1; // y: S, N (by1 only for GY == 1); z: D, U (bz1: GZ == 1)
             VT v[EV];
-            EpiIn<VT, W> in;
         };
         auto load_step = [&](int cc, Step &S) __attribute__((always_inline)) {
             const int64_t i = line + cc;
-            S.bs = ld_s ? xat(i + ldy_s) : zero;
-            S.bn = ld_n ? xat(i + ldy_n) : zero;
-            S.bd = ld_d ? xat(i + ldz_d) : zero;
-            S.bu = ld_u ? xat(i + ldz_u) : zero;
+            if constexpr (GY == 1) {
+                S.by0 = ld_s ? xat(i + ldy_s) : zero;
+                S.by1 = ld_n ? xat(i + ldy_n) : zero;
+            } else {
+                S.by0 = ld_s ? xat(i + ldy_s) : (ld_n ? xat(i + ldy_n) : zero);
+            }
+            if constexpr (GZ == 1) {
+                S.bz0 = ld_d ? xat(i + ldz_d) : zero;
+                S.bz1 = ld_u ? xat(i + ldz_u) : zero;
+            } else {
+                S.bz0 = ld_d ? xat(i + ldz_d) : (ld_u ? xat(i + ldz_u) : zero);
+            }
 #pragma unroll
             for (int q = 0; q < EV; ++q)
                 S.v[q] = (rowok && gl < LV) ? st7[(size_t)i * 8 + q * LV + gl] : VT(0);
-            if (rowok) {
-                const size_t o = (size_t)i * NV + k0;
-                epi_load<VT, NV, W, OP>(i, o, o, k0, S.in, yv, b, o1, isd, rr, out2, xupdv, last,
-                                        x_zero);
-            }
         };
         P xw = zero, xc = zero, xe = zero;
         if (rowok) {
@@ -965,32 +970,39 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st7(
             if (more)
                 load_step(cc + 1, nxt);
             // neighbours inside the wave's block from the lanes that own those lines
+            // epilogue operands of this row (issued before the cross-lane moves and the sums)
+            EpiIn<VT, W> in;
+            if (rowok) {
+                const size_t o = (size_t)(line + cc) * NV + k0;
+                epi_load<VT, NV, W, OP>(line + cc, o, o, k0, in, yv, b, o1, isd, rr, out2, xupdv,
+                                        last, x_zero);
+            }
             P ys, yn, zd, zu;
             if constexpr (GY == 1) {
-                ys = cur.bs;
-                yn = cur.bn;
+                ys = cur.by0;
+                yn = cur.by1;
             } else if constexpr (GY == 2) {
                 const P sib = shfl_pack<VT, W>(xc, lane ^ L);
-                ys = a == 0 ? cur.bs : sib;
-                yn = a == 0 ? sib : cur.bn;
+                ys = a == 0 ? cur.by0 : sib;
+                yn = a == 0 ? sib : cur.by0;
             } else {
                 const P sl = shfl_pack<VT, W>(xc, lane >= L ? lane - L : lane);
                 const P sh = shfl_pack<VT, W>(xc, lane + L < 64 ? lane + L : lane);
-                ys = a == 0 ? cur.bs : sl;
-                yn = a == GY - 1 ? cur.bn : sh;
+                ys = a == 0 ? cur.by0 : sl;
+                yn = a == GY - 1 ? cur.by0 : sh;
             }
             if constexpr (GZ == 1) {
-                zd = cur.bd;
-                zu = cur.bu;
+                zd = cur.bz0;
+                zu = cur.bz1;
             } else if constexpr (GZ == 2) {
                 const P sib = shfl_pack<VT, W>(xc, lane ^ (GY * L));
-                zd = bz == 0 ? cur.bd : sib;
-                zu = bz == 0 ? sib : cur.bu;
+                zd = bz == 0 ? cur.bz0 : sib;
+                zu = bz == 0 ? sib : cur.bz0;
             } else {
                 const P sl = shfl_pack<VT, W>(xc, lane >= GY * L ? lane - GY * L : lane);
                 const P sh = shfl_pack<VT, W>(xc, lane + GY * L < 64 ? lane + GY * L : lane);
-                zd = bz == 0 ? cur.bd : sl;
-                zu = bz == GZ - 1 ? cur.bu : sh;
+                zd = bz == 0 ? cur.bz0 : sl;
+                zu = bz == GZ - 1 ? cur.bz0 : sh;
             }
             VT vs[7];
             static_for<0, 7>([&](auto sc) {
@@ -1004,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st7(
                 VT acc[W];
 #pragma unroll
                 for (int w = 0; w < W; ++w)
-                    acc[w] = OP == OP_ADD ? cur.in.a.v[w] : VT(0);
+                    acc[w] = OP == OP_ADD ? in.a.v[w] : VT(0);
                 auto add = [&](const P &xv, VT v) __attribute__((always_inline)) {
 #pragma unroll
                     for (int w = 0; w < W; ++w)
@@ -1024,7 +1036,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_st7(
                 if (has_u)
                     add(zu, vs[6]);
                 const size_t o = (size_t)(line + cc) * NV + k0;
-                row_epilogue<VT, NV, W, OP>(o, o, acc, part, cur.in, yv, out2, out3, xupdv, c,
+                row_epilogue<VT, NV, W, OP>(o, o, acc, part, in, yv, out2, out3, xupdv, c,
                                             last, x_zero, out_f32);
             }
             xw = xc;

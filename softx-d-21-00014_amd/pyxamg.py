@@ -133,6 +133,8 @@ def _load():
         "xh_batch_autotune": [vp, C.c_int, vp],
         "xh_batch_set_tiles": [vp, vp],
         "xh_batch_set_persistent": [vp, C.c_int],
+        "xh_batch_set_layout": [vp, vp, vp],
+        "xh_batch_get_layout": [vp, vp, vp],
         "xh_batch_destroy": [vp],
         "xh_unet_create": [C.c_int, vp, u64, P(vp)],
         "xh_unet_destroy": [vp],
@@ -667,6 +669,18 @@ class Batch:
         check(hip().xh_batch_stats(self.h, s, C.byref(it), _ptr(conv), _ptr(a), _ptr(r),
                                    C.byref(lat)), "xh_batch_stats")
         return dict(iters=it.value, conv=conv, abs_res=a, rel_res=r, latency_ms=lat.value)
+
+    def layout(self):
+        """(tile passes, RHS per lane) per nv class 1, 2, 4, 8, 16"""
+        p = np.zeros(5, np.int32)
+        w = np.zeros(5, np.int32)
+        check(hip().xh_batch_get_layout(self.h, _ptr(p), _ptr(w)), "xh_batch_get_layout")
+        return [int(q) for q in p], [int(q) for q in w]
+
+    def set_layout(self, passes, rhs_per_lane):
+        p = np.ascontiguousarray(passes, np.int32)
+        w = np.ascontiguousarray(rhs_per_lane, np.int32)
+        check(hip().xh_batch_set_layout(self.h, _ptr(p), _ptr(w)), "xh_batch_set_layout")
 
     def set_persistent(self, on=True):
         """solve in one cooperative launch (True) or one launch sequence per iteration"""

@@ -120,6 +120,44 @@ def test_batch_single_launch_is_identical(gpu, O, method, precond):
         assert rel_diff(B.x(s), o["x"], nvs[s]) <= 1e-6
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_two_rhs_per_lane_layout(gpu, O, persistent):
+    # two RHS per lane (nv/2 lanes per row, twice the rows per pass) for every class with
+    # nv >= 2: each RHS's arithmetic is unchanged, only the dot partials group rows
+    # differently, so every system meets the oracle bar; graph and eager runs of one
+    # layout are identical
+    X = gpu
+    sizes = (6, 9, 12, 7, 10, 5)
+    nvs = (1, 2, 4, 8, 16, 4)
+    sy = systems(X, sizes, seed=60)
+    xs = []
+    for graph in (0, 1):
+        prm = params(X, graph=graph)
+        B = X.Batch([(r, c, v) for r, c, v, _ in sy], nvs, prm,
+                    lmax=[lv[0]["eig"][1] for _, _, _, lv in sy])
+        B.set_layout([4, 3, 2, 1, 2], [1, 2, 2, 2, 2])
+        assert B.layout() == ([4, 3, 2, 1, 2], [1, 2, 2, 2, 2])
+        if persistent:
+            B.set_persistent(True)
+        bs = []
+        for s, (r, _, _, _) in enumerate(sy):
+            b = X.rhs_uniform(70 + s, len(r) - 1, nvs[s])
+            B.set_rhs(s, b)
+            bs.append(b)
+        B.solve()
+        xs.append([B.x(s) for s in range(len(sizes))])
+    for a, b in zip(*xs):
+        assert np.array_equal(a, b)
+    for s, (r, c, v, lv) in enumerate(sy):
+        o = O.solve(O.Hier(lv), O.params_from(prm), nvs[s], bs[s])
+        st = B.stats(s)
+        assert st["conv"].all() and o["conv"].all()
+        assert abs(st["iters"] - o["iters"]) <= 1
+        assert rel_diff(B.x(s), o["x"], nvs[s]) <= 1e-6
+    with pytest.raises(X.XamgError):  # nv = 1 takes one RHS per lane only
+        B.set_layout([1, 1, 1, 1, 1], [2, 2, 2, 2, 2])
+
+
 def test_batch_rejects_unsupported(gpu):
     X = gpu
     sy = systems(X, (5,))
