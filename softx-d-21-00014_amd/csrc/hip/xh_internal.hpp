@@ -49,8 +49,12 @@ struct Mat {
     // slots {C, -z, -y, -x, +x, +y, +z, unused}, 0 where a neighbour is outside the grid
     void *st7 = nullptr;
     // core-block matrix of hybrid GS (entries inside each row's block of core_B rows only):
-    // core_B > 0, and core_emax = the largest entry count of one block
-    int64_t core_B = 0, core_emax = 0;
+    // core_B > 0, and core_emax = the largest entry count of one block.  Each row's entry
+    // list is padded to a multiple of core_kb (>= 1 batch) with pad entries of column =
+    // the block's end and value +0; nnz counts the real entries only; core_rowmax = the
+    // longest padded row.
+    int64_t core_B = 0, core_emax = 0, core_rowmax = 0;
+    int core_kb = 0;
     size_t bytes = 0;            // device bytes owned
 };
 

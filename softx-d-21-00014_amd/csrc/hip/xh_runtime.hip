@@ -437,10 +437,29 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
 // The core-block part of h for hybrid GS with blocks of B rows: row i keeps the entries
 // whose column lies in [B*(i/B), B*(i/B) + B), in stored order (the only ones the sweep of
 // csr_SGS on the core block reads, csr_gs.inl:75-130), with u32 columns and the values
-// in the matrix precision; inv_diag is taken from A (not owned).
+// in the matrix precision; inv_diag is taken from A (not owned).  Each row's list is
+// padded to a multiple of KB entries (at least KB) with column = the block's end and
+// value +0: the sweep kernel reads a zero x there, so a pad subtracts +0 * +0 = +0 from
+// the row sum, which leaves every value (-0 included) unchanged.  KB (3 or 4) is the width
+// that needs fewer pads.
 int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
     M = Mat();
     const int64_t n = h.nrows;
+    std::vector<int64_t> cnt(n, 0);
+    int64_t nreal = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t lo = (i / B) * B, hi = std::min(n, lo + B);
+        for (uint32_t j = h.row[i]; j < h.row[i + 1]; ++j)
+            cnt[i] += (int64_t)h.col[j] >= lo && (int64_t)h.col[j] < hi;
+        nreal += cnt[i];
+    }
+    auto padded = [](int64_t c, int kb) { return std::max<int64_t>(kb, (c + kb - 1) / kb * kb); };
+    int64_t tot3 = 0, tot4 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        tot3 += padded(cnt[i], 3);
+        tot4 += padded(cnt[i], 4);
+    }
+    const int KB = tot3 < tot4 ? 3 : 4;
     std::vector<uint32_t> row(n + 1, 0), col;
     std::vector<double> val;
     for (int64_t i = 0; i < n; ++i) {
@@ -450,11 +469,16 @@ int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
                 col.push_back(h.col[j]);
                 val.push_back(h.val[j]);
             }
+        for (int64_t q = cnt[i]; q < padded(cnt[i], KB); ++q) {
+            col.push_back((uint32_t)hi);
+            val.push_back(0.0);
+        }
         row[i + 1] = (uint32_t)col.size();
     }
     const int64_t nnz = (int64_t)col.size();
     M.nrows = M.ncols = n;
-    M.nnz = nnz;
+    M.nnz = nreal;
+    M.core_kb = KB;
     M.prec = prec;
     M.idx = IDX_U32;
     size_t acc = 0;
@@ -488,6 +512,8 @@ int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
     M.core_B = B;
     for (int64_t lo = 0; lo < n; lo += B)
         M.core_emax = std::max<int64_t>(M.core_emax, row[std::min(n, lo + B)] - row[lo]);
+    for (int64_t i = 0; i < n; ++i)
+        M.core_rowmax = std::max<int64_t>(M.core_rowmax, row[i + 1] - row[i]);
     M.bytes = acc;
     return 0;
 }

@@ -154,12 +154,15 @@ __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__r
 // slot): every load instruction holds an L1 request slot whether it hits or misses, so a
 // slot past the row end issues none (1-entry rows: 303 -> 217 us at 128^3, nv = 32);
 // the accumulation of a slot past the row end is skipped the same way.
-template <typename VT, int NV, int W>
+// SKIP: entries whose column lies in [slo, slo + sn) are skipped the same way (neither
+// loaded nor added), which keeps the order of the others.
+template <typename VT, int NV, int W, bool SKIP = false>
 __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chunk<VT, NV, W> &ch,
                                               uint32_t base, const VT *__restrict__ x, int k0,
-                                              VT (&acc)[W]) {
+                                              VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0) {
     using C = Chunk<VT, NV, W>;
     constexpr int BW = C::BW, L = C::L;
+    auto kept = [&](uint32_t cabs) { return !SKIP || cabs - slo >= sn; };
     if constexpr (L >= 2) {
         bool more = true; // uniform within the row's lane group
         static_for<0, C::CH / BW>([&](auto bc) {
@@ -178,12 +181,12 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
             Pack<VT, W> xv[BW];
 #pragma unroll
             for (int q = 0; q < BW; ++q)
-                if (j + b0 + q < e)
+                if (j + b0 + q < e && kept(base + c[q]))
                     xv[q] = ld<VT, W>(x + (size_t)(base + c[q]) * NV + k0);
             static_for<0, BW>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 constexpr int t = b0 + q;
-                if (j + t < e) {
+                if (j + t < e && kept(base + c[q])) {
                     const VT vq = bcast<L, t % C::LL>(ch.v[t / C::LL]);
 #pragma unroll
                     for (int w = 0; w < W; ++w)
@@ -200,20 +203,21 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
             xv[q] = ld<VT, W>(x + (size_t)(base + ch.c[q]) * NV + k0);
 #pragma unroll
         for (int q = 0; q < BW; ++q)
-            if (j + q < e)
+            if (j + q < e && kept(base + ch.c[q]))
 #pragma unroll
                 for (int w = 0; w < W; ++w)
                     acc[w] = acc[w] + xv[q].v[w] * ch.v[q];
     }
 }
 
-// the whole row [j, e), its first chunk already in `first`
-template <typename VT, typename IT, int NV, int W>
+// the whole row [j, e), its first chunk already in `first` (SKIP: without the columns
+// in [slo, slo + sn))
+template <typename VT, typename IT, int NV, int W, bool SKIP = false>
 __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
                                         const Chunk<VT, NV, W> &first,
                                         const IT *__restrict__ col, const VT *__restrict__ val,
                                         uint32_t base, const VT *__restrict__ x, int k0,
-                                        VT (&acc)[W]) {
+                                        VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0) {
     using C = Chunk<VT, NV, W>;
     // a longer row loads its next chunk before the x loads of the current one, so the
     // chunk's latency overlaps them
@@ -223,7 +227,7 @@ __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
         C nxt;
         if (more)
             load_chunk<VT, IT, NV, W>(j + C::CH, e, col, val, nxt);
-        consume_chunk<VT, NV, W>(j, e, cur, base, x, k0, acc);
+        consume_chunk<VT, NV, W, SKIP>(j, e, cur, base, x, k0, acc, slo, sn);
         if (!more)
             break;
         j += C::CH;
@@ -491,7 +495,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     tile_rows(sc, k, start, end);
     // Pipeline: the row pointers and panel base of the row two passes ahead and the
     // first chunk of the next row are loaded while this row's x loads are in flight.
-    constexpr bool kPrefetch = OP != OP_OFFBLOCK;
     int64_t i = start + tr;
     uint32_t jb = 0, je = 0, bs = 0, nb = 0, ne = 0, nbs = 0;
     if (i < end) {
@@ -505,8 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         nbs = panel_base<IT>(pbase, shift, i + RPB);
     }
     Chunk<VT, NV, W> ch;
-    if constexpr (kPrefetch)
-        load_chunk<VT, IT, NV, W>(jb, je, col, val, ch);
+    load_chunk<VT, IT, NV, W>(jb, je, col, val, ch);
     // Drain the prologue loads once per tile: the row loop's head then merges only its
     // own back edge, where the last y store may stay in flight.
     __builtin_amdgcn_s_waitcnt(0);
@@ -514,8 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         const size_t o = (size_t)i * NV + k0;
         const uint32_t base = bs;
         Chunk<VT, NV, W> nch;
-        if constexpr (kPrefetch)
-            load_chunk<VT, IT, NV, W>(nb, ne, col, val, nch);
+        load_chunk<VT, IT, NV, W>(nb, ne, col, val, nch);
         uint32_t nnb = 0, nne = 0, nnbs = 0;
         if (i + 2 * RPB < end) {
             nnb = row[i + 2 * RPB];
@@ -539,16 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         }
         if constexpr (OP == OP_OFFBLOCK) {
             // t = b - sum over off-block columns (old x), HSGS core blocks of blk_rows
-            const int64_t lo = (i / blk_rows) * blk_rows, hi = lo + blk_rows;
-            for (uint32_t j = jb; j < je; ++j) {
-                const int64_t cc = (int64_t)(base + (uint32_t)col[j]);
-                if (cc < lo || cc >= hi) {
-                    const P xv = ld<VT, W>(x + (size_t)cc * NV + k0);
-#pragma unroll
-                    for (int w = 0; w < W; ++w)
-                        acc[w] = acc[w] + xv.v[w] * val[j];
-                }
-            }
+            const uint32_t lo = (uint32_t)((i / blk_rows) * blk_rows);
+            row_acc<VT, IT, NV, W, true>(jb, je, ch, col, val, base, x, k0, acc, lo,
+                                         (uint32_t)blk_rows);
         } else {
             row_acc<VT, IT, NV, W>(jb, je, ch, col, val, base, x, k0, acc);
         }
@@ -562,8 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         nb = nnb;
         ne = nne;
         nbs = nnbs;
-        if constexpr (kPrefetch)
-            ch = nch;
+        ch = nch;
     }
     }
     if constexpr (ND > 0)

@@ -322,12 +322,63 @@ __global__ __launch_bounds__(64) void k_hsgs_lds(const uint32_t *__restrict__ ro
         x[(size_t)i * NV + k] = xs[(i - lo) * NV + k];
 }
 
+// Copies count elements from global memory into LDS with the lanes lane, lane + nl, ...;
+// U loads are issued before their LDS writes so that a copy costs count / (nl * U) memory
+// round trips instead of one per element.  16-byte vectors where both ends are aligned.
+// Loads and stores past the end re-read and re-write the last element (always in
+// bounds, same value), so that no load can be moved behind a branch and the batch stays
+// in flight together.
+template <typename VT, int U>
+__device__ inline void stage_copy(VT *__restrict__ dst, const VT *__restrict__ src,
+                                  int64_t count, int lane, int nl) {
+    int64_t done_el = 0;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+        const int64_t nvec = count * (int64_t)sizeof(VT) / 16;
+        const uint4 *s4 = (const uint4 *)src;
+        uint4 *d4 = (uint4 *)dst;
+        for (int64_t b = lane; b < nvec; b += (int64_t)nl * U) {
+            uint4 r[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = b + (int64_t)u * nl;
+                r[u] = s4[i < nvec ? i : nvec - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = b + (int64_t)u * nl;
+                d4[i < nvec ? i : nvec - 1] = r[u];
+            }
+        }
+        done_el = nvec * 16 / (int64_t)sizeof(VT);
+    }
+    for (int64_t b = done_el + lane; b < count; b += (int64_t)nl * U) {
+        VT r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = b + (int64_t)u * nl;
+            r[u] = src[i < count ? i : count - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = b + (int64_t)u * nl;
+            dst[i < count ? i : count - 1] = r[u];
+        }
+    }
+}
+
 // The same sweeps on a core-block matrix (only entries inside the row's block, u32
-// columns): a group first stages its block's row pointers, local columns and values in
-// LDS with x, then sweeps from LDS; the next row's t and inverse diagonal are loaded one
-// row ahead.  Each lane reads and writes only its own RHS column of the staged x, so the
-// sweep needs no synchronisation.  Entry order and arithmetic as k_hsgs_lds.
-template <typename VT, int NV>
+// columns; each row padded to whole batches of KB entries, pads pointing at the block's
+// end with value +0).  A group first stages its block's x, row pointers, local columns,
+// values and inverse diagonal in LDS (loads issued in batches), with one more x row of
+// zeros at local row nr for the pads, then sweeps from LDS.  In the sweep, t of the next R
+// rows is loaded while the current R rows are swept, and the next row's row pointers,
+// inverse diagonal and first KB columns and values are read from LDS while the current
+// row computes; the x operands of a row are read after the previous row's x was written
+// (LDS accesses of one wave are ordered).  Each lane reads and writes only its own RHS
+// column of x during the sweep.  Entry order and arithmetic as k_hsgs_lds: the products
+// x_c * a_ic are rounded, then subtracted from t one by one in stored order; a pad
+// subtracts +0 * +0 = +0, which changes no value.  ONE: every row is one batch.
+template <typename VT, int NV, int KB, bool ONE>
 __global__ __launch_bounds__(64) void k_hsgs_core(const uint32_t *__restrict__ row,
                                                   const uint32_t *__restrict__ col,
                                                   const VT *__restrict__ val,
@@ -338,47 +389,147 @@ __global__ __launch_bounds__(64) void k_hsgs_core(const uint32_t *__restrict__ r
                                                   const int *__restrict__ done) {
     if (done != nullptr && *done)
         return;
+    constexpr int R = 16;
     extern __shared__ unsigned char smem[];
     constexpr int G = 64 / NV;
     const int g = threadIdx.x / NV, k = threadIdx.x % NV;
-    VT *xs = (VT *)(smem + (size_t)g * gbytes);
-    VT *cv = xs + B * NV;
-    uint32_t *cc = (uint32_t *)(cv + E);
-    uint32_t *rp = cc + E;
+    // per group: x [B + 1][NV], values [E], inverse diagonal [B] (VT); local columns [E],
+    // row pointers [B + 1] (u32)
+    VT *__restrict__ xs = (VT *)(smem + (size_t)g * gbytes);
+    VT *__restrict__ cv = xs + (B + 1) * NV;
+    VT *__restrict__ ivs = cv + E;
+    uint32_t *__restrict__ cc = (uint32_t *)(ivs + B);
+    uint32_t *__restrict__ rp = cc + E;
     const int64_t lo = ((int64_t)blockIdx.x * G + g) * B;
     const bool active = lo < n;
-    const int64_t hi = active ? (lo + B < n ? lo + B : n) : lo;
-    const int64_t nr = hi - lo;
+    const int nr = active ? (int)(lo + B < n ? B : n - lo) : 0;
     const uint32_t e0 = active ? row[lo] : 0u;
-    const uint32_t ne = active ? row[hi] - e0 : 0u;
-    for (int64_t q = k; q <= nr && active; q += NV)
-        rp[q] = row[lo + q] - e0;
-    for (uint32_t q = k; q < ne; q += NV) {
-        cc[q] = col[e0 + q] - (uint32_t)lo;
-        cv[q] = val[e0 + q];
-    }
-    for (int64_t q = 0; q < nr; ++q)
-        xs[q * NV + k] = x_zero ? VT(0) : x[(size_t)(lo + q) * NV + k];
-    __syncthreads();
-    for (int sweep = 0; sweep < 2 && nr > 0; ++sweep) {
-        int64_t li = sweep == 0 ? 0 : nr - 1;
-        VT tn = t[(size_t)(lo + li) * NV + k], dn = invd[lo + li];
-        for (int64_t q = 0; q < nr; ++q) {
-            const int64_t lc = li;
-            VT sum = tn;
-            const VT di = dn;
-            if (q + 1 < nr) {
-                li = sweep == 0 ? q + 1 : nr - 2 - q;
-                tn = t[(size_t)(lo + li) * NV + k];
-                dn = invd[lo + li];
+    const uint32_t ne = active ? row[lo + nr] - e0 : 0u;
+    if (active) {
+        for (int b = k; b <= nr; b += NV * 8) {
+            uint32_t r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                r[u] = row[lo + (b + u * NV <= nr ? b + u * NV : nr)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                rp[b + u * NV <= nr ? b + u * NV : nr] = r[u] - e0;
+        }
+        for (uint32_t b = k; b < ne; b += NV * 8u) {
+            uint32_t c[8];
+            VT v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t i = b + u * NV < ne ? b + u * NV : ne - 1;
+                c[u] = col[e0 + i];
+                v[u] = val[e0 + i];
             }
-            for (uint32_t j = rp[lc]; j < rp[lc + 1]; ++j)
-                sum = sum - xs[(size_t)cc[j] * NV + k] * cv[j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t i = b + u * NV < ne ? b + u * NV : ne - 1;
+                cc[i] = c[u] - (uint32_t)lo;
+                cv[i] = v[u];
+            }
+        }
+        stage_copy<VT, 8>(ivs, invd + lo, nr, k, NV);
+        if (x_zero)
+            for (int q = 0; q < nr; ++q)
+                xs[q * NV + k] = VT(0);
+        else
+            stage_copy<VT, 12>(xs, x + (size_t)lo * NV, (int64_t)nr * NV, k, NV);
+        xs[nr * NV + k] = VT(0);
+    }
+    __syncthreads();
+    const VT *__restrict__ tb = t + (size_t)lo * NV + k;
+    for (int sweep = 0; sweep < 2 && nr > 0; ++sweep) {
+        // step q of the sweep works on local row r0 + dir * q
+        const int dir = sweep == 0 ? 1 : -1, r0 = sweep == 0 ? 0 : nr - 1;
+        // the same, with steps past the end clamped to the last one
+        auto row_of = [&](int q) { return r0 + dir * (q < nr ? q : nr - 1); };
+        VT tc[R], tn[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+            tc[u] = tb[row_of(u) * NV];
+        int lc = r0;
+        uint32_t j0 = rp[lc], j1 = rp[lc + 1];
+        VT di = ivs[lc];
+        uint32_t c4[KB];
+        VT v4[KB];
+#pragma unroll
+        for (int v = 0; v < KB; ++v) {
+            c4[v] = cc[j0 + v];
+            v4[v] = cv[j0 + v];
+        }
+        // sweeps the row lc with t value tv, then moves to row ln
+        auto step = [&](const VT tv, const int ln) {
+            VT sum = tv;
+            VT p[KB];
+#pragma unroll
+            for (int v = 0; v < KB; ++v)
+                p[v] = xs[c4[v] * NV + k] * v4[v];
+            const VT xo = xs[lc * NV + k];
+            // the next row's descriptor: independent of x
+            const uint32_t nj0 = rp[ln], nj1 = rp[ln + 1];
+            const VT ndi = ivs[ln];
+            uint32_t nc4[KB];
+            VT nv4[KB];
+#pragma unroll
+            for (int v = 0; v < KB; ++v) {
+                nc4[v] = cc[nj0 + v];
+                nv4[v] = cv[nj0 + v];
+            }
+#pragma unroll
+            for (int v = 0; v < KB; ++v)
+                sum = sum - p[v];
+            // rows of more than one batch (ONE: no row has more than KB entries)
+            for (uint32_t j = j0 + KB; !ONE && j < j1; j += KB) {
+                VT pp[KB];
+#pragma unroll
+                for (int v = 0; v < KB; ++v)
+                    pp[v] = xs[cc[j + v] * NV + k] * cv[j + v];
+#pragma unroll
+                for (int v = 0; v < KB; ++v)
+                    sum = sum - pp[v];
+            }
             const VT coeff = relax * di;
-            xs[lc * NV + k] = xs[lc * NV + k] + coeff * sum;
+            xs[lc * NV + k] = xo + coeff * sum;
+            lc = ln;
+            j0 = nj0;
+            j1 = nj1;
+            di = ndi;
+#pragma unroll
+            for (int v = 0; v < KB; ++v) {
+                c4[v] = nc4[v];
+                v4[v] = nv4[v];
+            }
+        };
+        for (int q0 = 0; q0 < nr; q0 += R) {
+            // t of the next R steps, in flight while these R rows are swept
+            if (q0 + 2 * R <= nr) {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    tn[u] = tb[(r0 + dir * (q0 + R + u)) * NV];
+            } else {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    tn[u] = tb[row_of(q0 + R + u) * NV];
+            }
+            if (q0 + R <= nr) {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    step(tc[u], u + 1 < R ? r0 + dir * (q0 + u + 1) : row_of(q0 + u + 1));
+            } else {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    if (q0 + u < nr)
+                        step(tc[u], row_of(q0 + u + 1));
+            }
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                tc[u] = tn[u];
         }
     }
-    for (int64_t q = 0; q < nr; ++q)
+    for (int q = 0; q < nr; ++q)
         x[(size_t)(lo + q) * NV + k] = xs[q * NV + k];
 }
 
@@ -532,11 +683,17 @@ const char *hsgs_t(const Mat &A, const void *t, void *x, int x_zero, double rela
     const int grid = (int)((nblk + G - 1) / G);
     const size_t lds = (size_t)64 * B * sizeof(VT);
     if constexpr (sizeof(IT) == 4) {
-        // a core-block matrix of this block size whose blocks fit in LDS: staged sweeps
-        const size_t gb = ((size_t)B * NV * sizeof(VT) + (size_t)A.core_emax * (sizeof(VT) + 4) +
+        // a padded core-block matrix of this block size whose blocks fit in LDS: staged
+        // sweeps.  Per group: x with its zero row, values and inverse diagonal (VT), then
+        // columns and row pointers (u32)
+        const size_t gb = ((size_t)(B + 1) * NV * sizeof(VT) +
+                           (size_t)A.core_emax * (sizeof(VT) + 4) + (size_t)B * sizeof(VT) +
                            (size_t)(B + 1) * 4 + 15) / 16 * 16;
-        if (A.core_B == B && G * gb <= 64 * 1024) {
-            hipLaunchKernelGGL((k_hsgs_core<VT, NV>), dim3(grid), dim3(64), G * gb, s, A.row,
+        if (A.core_B == B && (A.core_kb == 3 || A.core_kb == 4) && G * gb <= 64 * 1024) {
+            const bool one = A.core_rowmax <= A.core_kb;
+            auto kern = A.core_kb == 3 ? (one ? k_hsgs_core<VT, NV, 3, true> : k_hsgs_core<VT, NV, 3, false>)
+                                       : (one ? k_hsgs_core<VT, NV, 4, true> : k_hsgs_core<VT, NV, 4, false>);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), G * gb, s, A.row,
                                (const uint32_t *)A.col, (const VT *)A.val,
                                (const VT *)A.inv_diag, A.nrows, B, A.core_emax, gb,
                                (const VT *)t, (VT *)x, x_zero, (VT)relax, done);
