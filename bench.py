@@ -365,6 +365,9 @@ def main():
         st_ = [Bt.stats(i) for i in range(64)]
         lat = np.array([q["latency_ms"] for q in st_])
         ms_b = (b1 - b0) * 1e3 / args.steps
+        # the preconditioner's batched Chebyshev step (14 of the 18 SpMVs of an
+        # iteration) over every system, timed on its own
+        us_sp, bytes_sp = Bt.spmv_timed(reps=20)
         single = None
         if args.batch_single_launch:
             # the same batch as one cooperative launch per solve (device-side iterations)
@@ -403,6 +406,11 @@ def main():
             "all_converged": bool(all(q["conv"].all() for q in st_)),
             "tile_passes_nv_1_2_4_8_16": [int(t) for t in tiles],
             "rhs_per_lane_nv_1_2_4_8_16": Bt.layout()[1],
+            "spmv": {"kernel": "k_batch<B_CHEB> (every system active)",
+                     "us_per_launch": round(us_sp, 1),
+                     "algorithmic_bytes": int(bytes_sp),
+                     "gb_s": round(bytes_sp / (us_sp * 1e-6) / 1e9, 1),
+                     "frac": round(bytes_sp / (us_sp * 1e-6) / HBM_PEAK, 4)},
             "single_launch": single,
         }
         del Bt

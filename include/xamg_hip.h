@@ -335,6 +335,12 @@ int xh_batch_stats(xh_batch B, int s, int32_t *iters, int32_t *conv, double *abs
  * xh_batch_get_layout reads both.  The layout changes only the grouping of the dot
  * partial sums, not any RHS's arithmetic. */
 int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes);
+/* Times `reps` launches of the batched Chebyshev step (y = D^-1/2 (c r + D^-1/2 A u), the
+ * preconditioner's SpMV) over every work item with every system active; returns the mean
+ * microseconds per launch and its algorithmic bytes (matrix, row pointers, D^-1/2, work
+ * items, u and r read, y written once).  Saves and restores the solver state; overwrites
+ * a work vector, so call it between solves.  No reference counterpart (measurement). */
+int xh_batch_spmv_timed(xh_batch B, int reps, double *us_per_launch, double *alg_bytes);
 int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes);
 int xh_batch_set_layout(xh_batch B, const int32_t *tile_passes, const int32_t *rhs_per_lane);
 int xh_batch_get_layout(xh_batch B, int32_t *tile_passes, int32_t *rhs_per_lane);

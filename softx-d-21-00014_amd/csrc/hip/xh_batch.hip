@@ -91,12 +91,38 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
 #pragma unroll
                 for (int u = 0; u < W; ++u)
                     acc[u] = 0.0;
-                for (uint32_t j = d.row[g]; j < d.row[g + 1]; ++j) {
-                    const size_t xo = (size_t)d.col[j] * nv + k0;
-                    const double a = d.val[j];
+                // entries in batches of EB: columns and values of the batch, then its x
+                // rows, all in flight together; slots past the row end re-read the last
+                // entry (in bounds) and are not added, so the sum keeps the stored order
+                constexpr int EB = 4;
+                const uint32_t j1 = d.row[g + 1];
+                for (uint32_t j = d.row[g]; j < j1; j += EB) {
+                    uint32_t c[EB];
+                    double a[EB], xv[EB][W];
 #pragma unroll
-                    for (int u = 0; u < W; ++u)
-                        acc[u] = acc[u] + xs[xo + u] * a;
+                    for (int q = 0; q < EB; ++q) {
+                        const uint32_t jj = j + q < j1 ? j + q : j1 - 1;
+                        c[q] = d.col[jj];
+                        a[q] = d.val[jj];
+                    }
+#pragma unroll
+                    for (int q = 0; q < EB; ++q) {
+                        const double *xp = xs + (size_t)c[q] * nv + k0;
+                        if constexpr (W == 2) {
+                            // 16-B aligned: system vectors start on 256 B, k0 and nv even
+                            const double2 t2 = *(const double2 *)xp;
+                            xv[q][0] = t2.x;
+                            xv[q][1] = t2.y;
+                        } else {
+                            xv[q][0] = xp[0];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < EB; ++q)
+                        if (j + q < j1)
+#pragma unroll
+                            for (int u = 0; u < W; ++u)
+                                acc[u] = acc[u] + xv[q][u] * a[q];
                 }
 #pragma unroll
                 for (int u = 0; u < W; ++u) {

@@ -2444,6 +2444,7 @@ extern "C" int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nv
         B->n.push_back(n);
         B->nv.push_back(v);
         B->rbase.push_back(rows);
+        ve = (ve + 31) / 32 * 32; // each system's vectors start on 256 B (16-B vector loads)
         B->voff.push_back(ve);
         rows += n;
         nnz += z;
@@ -2752,6 +2753,55 @@ extern "C" int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes) {
         for (int c = 0; c < 5; ++c)
             tile_passes[c] = B->tile_passes[c];
     return upload_items(B);
+}
+
+// Times the batched Chebyshev step (B_CHEB: y = isd (c r + isd (A u)) over every work item
+// of every system, as in the preconditioner) with every system active: reps launches
+// between two events on the batch stream.  The systems' solver state (device and its host
+// copy) is saved first and restored afterwards; the work vector t is overwritten.
+extern "C" int xh_batch_spmv_timed(xh_batch B, int reps, double *us_per_launch,
+                                   double *alg_bytes) {
+    if (!B || reps < 1 || !us_per_launch || !alg_bytes)
+        return fail(XH_EINVAL, "xh_batch_spmv_timed: bad arguments");
+    hipStream_t st = B->stream;
+    XH_HIP(hipStreamSynchronize(st));
+    std::vector<SolverState> saved(B->nsys), saved_host(B->hst, B->hst + B->nsys);
+    XH_HIP(hipMemcpy(saved.data(), B->st, sizeof(SolverState) * B->nsys, hipMemcpyDeviceToHost));
+    for (int s = 0; s < B->nsys; ++s) {
+        std::memset(&B->hst[s], 0, sizeof(SolverState));
+        B->hst[s].nv = B->nv[s];
+    }
+    XH_HIP(hipMemcpy(B->st, B->hst, sizeof(SolverState) * B->nsys, hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    XH_HIP(hipEventCreate(&e0));
+    XH_HIP(hipEventCreate(&e1));
+    BatchVecs v;
+    v.in = B->b;
+    v.o1 = B->b;
+    v.out = B->t;
+    const BatchDev d = batch_dev(B);
+    const int nit = (int)B->items.size();
+    batch_launch(B_CHEB, d, B->d_items, nit, B->grid, v, 0, 0, nullptr, st);
+    XH_HIP(hipEventRecord(e0, st));
+    for (int q = 0; q < reps; ++q)
+        batch_launch(B_CHEB, d, B->d_items, nit, B->grid, v, 0, 0, nullptr, st);
+    XH_HIP(hipEventRecord(e1, st));
+    XH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    XH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    XH_HIP(hipMemcpy(B->st, saved.data(), sizeof(SolverState) * B->nsys, hipMemcpyHostToDevice));
+    std::copy(saved_host.begin(), saved_host.end(), B->hst);
+    double vec = 0.0;
+    for (int s = 0; s < B->nsys; ++s)
+        vec += (double)B->n[s] * B->nv[s] * 8.0;
+    // matrix values and columns, row pointers, inverse square-root diagonal, the work
+    // items; u and r read, y written
+    *alg_bytes = (double)B->nnz * 12.0 + (double)B->rows * (4.0 + 8.0) +
+                 (double)nit * sizeof(BatchItem) + 3.0 * vec;
+    *us_per_launch = 1000.0 * ms / reps;
+    return 0;
 }
 
 extern "C" int xh_batch_set_tiles(xh_batch B, const int32_t *tile_passes) {

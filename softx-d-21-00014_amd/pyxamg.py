@@ -131,6 +131,7 @@ def _load():
         "xh_batch_get_x": [vp, C.c_int, vp],
         "xh_batch_stats": [vp, C.c_int, P(i32), vp, vp, vp, P(f64)],
         "xh_batch_autotune": [vp, C.c_int, vp],
+        "xh_batch_spmv_timed": [vp, C.c_int, P(f64), P(f64)],
         "xh_batch_set_tiles": [vp, vp],
         "xh_batch_set_persistent": [vp, C.c_int],
         "xh_batch_set_layout": [vp, vp, vp],
@@ -694,6 +695,14 @@ class Batch:
     def set_tiles(self, tiles):
         t = np.ascontiguousarray(tiles, np.int32)
         check(hip().xh_batch_set_tiles(self.h, _ptr(t)), "xh_batch_set_tiles")
+
+    def spmv_timed(self, reps=20):
+        """(µs per launch, algorithmic bytes) of the batched Chebyshev step over every
+        system (xh_batch_spmv_timed); call between solves"""
+        us, nb = C.c_double(), C.c_double()
+        check(hip().xh_batch_spmv_timed(self.h, reps, C.byref(us), C.byref(nb)),
+              "xh_batch_spmv_timed")
+        return us.value, nb.value
 
 
 class UNet:

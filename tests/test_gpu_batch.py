@@ -91,6 +91,26 @@ def test_batch_graph_eager_and_retiling(gpu):
     assert all(B.stats(s)["conv"].all() for s in range(len(sizes)))
 
 
+def test_batch_spmv_timed_keeps_the_solve(gpu):
+    # the timed batched Chebyshev step reports its algorithmic bytes, restores the solver
+    # state, and a solve after it gives the same solution as before it
+    X = gpu
+    sizes, nvs = (8, 13, 5, 11), (4, 1, 16, 8)
+    sy = systems(X, sizes, seed=30)
+    B, _ = run_batch(X, sy, nvs, params(X))
+    xs = [B.x(s) for s in range(len(sizes))]
+    its = [B.stats(s)["iters"] for s in range(len(sizes))]
+    us, nbytes = B.spmv_timed(reps=3)
+    nnz = sum(len(c) for _, c, _, _ in sy)
+    rows = sum(N ** 3 for N in sizes)
+    vec = sum(N ** 3 * nv * 8 for N, nv in zip(sizes, nvs))
+    assert us > 0 and nbytes >= nnz * 12 + rows * 12 + 3 * vec
+    assert [B.stats(s)["iters"] for s in range(len(sizes))] == its
+    B.solve()
+    for s in range(len(sizes)):
+        assert np.array_equal(B.x(s), xs[s])
+
+
 @pytest.mark.parametrize("method,precond", [("PBiCGStab", "Chebyshev"), ("BiCGStab", None),
                                             ("MergedPBiCGStab", "Identity")])
 def test_batch_single_launch_is_identical(gpu, O, method, precond):
