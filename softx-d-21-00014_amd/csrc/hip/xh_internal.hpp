@@ -155,6 +155,9 @@ int64_t csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32
                  int64_t min_fold);
 
 // Launch-shape knobs (xh_set_option)
+// rows of at least this many entries on average take the structured-grid SpMV by default
+constexpr int64_t kGridMinRowNnz = 12;
+
 struct Tuning {
     int64_t strip_budget = 8 << 20; // bytes of x+y+A touched between reuses of an x row
     int tile_passes = 8;            // tile rows = tile_passes * rows per workgroup pass
@@ -162,10 +165,11 @@ struct Tuning {
     int dot_groups = 1024;          // workgroups of a launch that writes dot partials
     int fold_enable = 1;            // 0: always contiguous order
     int64_t fold_min_bytes = 2 << 20; // fold only when 2*fold*nv*sizeof(value) exceeds this
-    // 1: structured-grid matrices run the LDS-window line sweep (k_spmv_grid).  Off by
-    // default: at 128^3 it matches the row-parallel kernel at nv = 32 and is slower at
-    // other nv and in fp32
-    int grid_enable = 0;
+    // structured-grid matrices run the LDS-window line sweep (k_spmv_grid): 0 never, 1
+    // always, 2 (default) when rows average at least kGridMinRowNnz entries.  The 7-point
+    // cube at 128^3 only matches the row-parallel kernel at nv = 32 (slower at other nv
+    // and in fp32); the 27-point operator at 96^3 gains 10-20 %
+    int grid_enable = 2;
     int grid_items = 2048;          // target work items (tile x segment) per launch
     int grid_min_seg = 32;          // shortest x segment of a work item
     // 1: matrices with a complete 7-point stencil view run the register-window line
@@ -175,6 +179,12 @@ struct Tuning {
     int st7_min_seg = 32;           // shortest x segment of a work item
 };
 extern Tuning g_tune;
+
+// whether the structured-grid SpMV takes a matrix that has a grid view (spmv_grid_enable)
+inline bool grid_on(const Mat &A) {
+    return g_tune.grid_enable == 1 ||
+           (g_tune.grid_enable == 2 && A.nnz >= kGridMinRowNnz * A.nrows);
+}
 
 // ---- launch accounting (xh_solver_profile) ----
 // While a profile solve runs, every launch helper of the solve path brackets its kernel

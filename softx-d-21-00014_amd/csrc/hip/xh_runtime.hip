@@ -1912,7 +1912,7 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.fold_enable = (int)value;
     else if (k == "spmv_fold_min_bytes" && value >= 0)
         g_tune.fold_min_bytes = value;
-    else if (k == "spmv_grid_enable" && (value == 0 || value == 1))
+    else if (k == "spmv_grid_enable" && value >= 0 && value <= 2)
         g_tune.grid_enable = (int)value;
     else if (k == "spmv_st7_enable" && (value == 0 || value == 1))
         g_tune.st7_enable = (int)value;

@@ -32,6 +32,26 @@ def test_q1fem27_lognormal_hsgs(gpu, O, mg_prec):
         assert rel_diff(g["x"], x64, nv) <= 1e-6
 
 
+@pytest.mark.parametrize("mg_prec", [0, 2])
+def test_q1fem27_hsgs_preconditioner_bit_exact(gpu, O, mg_prec):
+    # one V-cycle with hybrid GS on the 27-point operator, whose finest residuals take the
+    # structured-grid path by default (the launches show it): bit-identical to the oracle's
+    X = gpu
+    N, nv = 24, 8
+    row, col, val, _ = X.q1fem27(N, sigma=2.0, seed=1)
+    levels = X.Amg(row, col, val, agg_num_levels=2, num_paths=2, coarse_size=500).export()
+    prm = X.default_params(method="MergedPBiCGStab", smoother="HSGS", sweeps=1, mg_prec=mg_prec)
+    S = X.Solver(X.Hier(levels), prm, nv)
+    b = X.rhs_uniform(9, N ** 3, nv)
+    db, dx = X.DeviceArray.from_host(b), X.DeviceArray(N ** 3 * nv)
+    S.precond_apply(db, dx)
+    assert np.array_equal(dx.to_host(), O.mg_apply(O.Hier(levels), O.params_from(prm), nv, b))
+    dx.zero()
+    names = {k["kernel"] for k in S.profile(db, dx, 1)}
+    prec = "double" if mg_prec == 0 else "float"
+    assert f"k_spmv_grid<{prec},{nv},OP_RESID>" in names, names
+
+
 def test_varcoef7_chebyshev8_without_amg(gpu, O):
     X = gpu
     N, nv = 20, 4

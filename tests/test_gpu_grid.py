@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 def grid_on(gpu):
     gpu.set_option("spmv_grid_enable", 1)
     yield gpu
-    gpu.set_option("spmv_grid_enable", 0)
+    gpu.set_option("spmv_grid_enable", 2)  # the default: by row length
 
 
 def _mats(X):
@@ -46,6 +46,24 @@ def test_grid_spmv_bit_identical(grid_on, prec):
             X.set_option("spmv_grid_enable", 1)
             assert np.array_equal(out[0][0], out[1][0]), (name, nv)
             assert np.array_equal(out[0][1], out[1][1]), (name, nv)
+
+
+def test_grid_path_by_row_length(gpu):
+    # the default takes the grid path for the 27-point operator (27 entries per interior
+    # row) and the row-parallel kernel for the 7-point cube: the launches of a solve show it
+    X = gpu
+    X.set_option("spmv_grid_enable", 2)
+    nv = 8
+    for name, (row, col, val), want in (("q1fem27", X.q1fem27(12, 12, 12)[:3], True),
+                                        ("cube", X.cube(12), False)):
+        levels = X.Amg(row, col, val).export()
+        prm = X.default_params(method="MergedPBiCGStab", smoother="Chebyshev", poly_order=2)
+        S = X.Solver(X.Hier(levels), prm, nv)
+        n = len(row) - 1
+        db = X.DeviceArray.from_host(X.rhs_uniform(3, n, nv))
+        dx = X.DeviceArray(n * nv)
+        names = {k["kernel"] for k in S.profile(db, dx, 1)}
+        assert any(k.startswith("k_spmv_grid<") for k in names) == want, (name, names)
 
 
 def test_no_grid_view_for_partial_grids(gpu):
