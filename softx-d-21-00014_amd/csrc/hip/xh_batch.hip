@@ -47,7 +47,7 @@ __device__ __forceinline__ void item_partials(const double (&part)[3][W], int nd
 template <int OP, int W>
 __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w,
                                            const BatchVecs &v, int ci, int last,
-                                           double *__restrict__ partials, int it) {
+                                           double *__restrict__ partials) {
     constexpr int ND = (OP == B_RESID0 || OP == B_DOT1 || OP == B_XR_UPDATE) ? 1
                        : (OP == B_DOT3)                                      ? 3
                                                                              : 0;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
             }
         }
     if constexpr (ND > 0)
-        item_partials<W>(part, ND, nv, partials + (size_t)it * kBatchPart);
+        item_partials<W>(part, ND, nv, partials + (size_t)w.slot * kBatchPart);
 }
 
 // one batched operation over the work items wg, wg + nwg, ... (one item per workgroup at
@@ -203,17 +203,50 @@ __device__ __forceinline__ void batch_phase(const BatchDev &d,
         if (d.st[w.sys].done)
             continue; // uniform over the workgroup: one item per workgroup at a time
         if (w.wpl == 2)
-            batch_item<OP, 2>(d, w, v, ci, last, partials, it);
+            batch_item<OP, 2>(d, w, v, ci, last, partials);
         else
-            batch_item<OP, 1>(d, w, v, ci, last, partials, it);
+            batch_item<OP, 1>(d, w, v, ci, last, partials);
     }
 }
 
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_batch(BatchDev d, const BatchItem *__restrict__ items,
-                                                  int nitems, BatchVecs v, int ci, int last,
+                                                  int nitems, const int *__restrict__ nact,
+                                                  BatchVecs v, int ci, int last,
                                                   double *__restrict__ partials) {
-    batch_phase<OP>(d, items, nitems, v, ci, last, partials, blockIdx.x, gridDim.x);
+    const int n = nact != nullptr ? min(*nact, nitems) : nitems;
+    batch_phase<OP>(d, items, n, v, ci, last, partials, blockIdx.x, gridDim.x);
+}
+
+// Work list of the systems not yet done: workgroup s copies system s's items (when it is
+// not done) to the position after the items of the active systems before it; workgroup 0
+// writes the total.  Every workgroup reads the same done flags (set by earlier launches).
+__global__ __launch_bounds__(256) void k_batch_compact(const BatchItem *__restrict__ items,
+                                                       const int *__restrict__ item_start,
+                                                       const int *__restrict__ item_count,
+                                                       const SolverState *__restrict__ st,
+                                                       int nsys, BatchItem *__restrict__ out,
+                                                       int *__restrict__ nact) {
+    const int s = blockIdx.x;
+    __shared__ int s_off;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int q = 0; q < s; ++q)
+            off += st[q].done ? 0 : item_count[q];
+        s_off = off;
+        if (s == 0) {
+            int tot = 0;
+            for (int q = 0; q < nsys; ++q)
+                tot += st[q].done ? 0 : item_count[q];
+            *nact = tot;
+        }
+    }
+    __syncthreads();
+    if (st[s].done)
+        return;
+    const int b = item_start[s], c = item_count[s], off = s_off;
+    for (int q = threadIdx.x; q < c; q += blockDim.x)
+        out[off + q] = items[b + q];
 }
 
 __global__ __launch_bounds__(256) void k_batch_finalize(int op, SolverState *__restrict__ st,
@@ -437,13 +470,15 @@ hipError_t batch_solve_launch(const BatchSolve &P, int grid, hipStream_t s) {
 }
 
 void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
-                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s) {
+                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s,
+                  const int *nact) {
     if (nitems == 0)
         return;
     const dim3 g(std::max(1, std::min(grid, nitems))), b(kBlock);
 #define XH_BATCH_CASE(OPV)                                                                       \
     case OPV:                                                                                    \
-        hipLaunchKernelGGL(k_batch<OPV>, g, b, 0, s, d, items, nitems, v, ci, last, partials);   \
+        hipLaunchKernelGGL(k_batch<OPV>, g, b, 0, s, d, items, nitems, nact, v, ci, last,       \
+                           partials);                                                            \
         break;
     switch (op) {
         XH_BATCH_CASE(B_RESID0)
@@ -463,6 +498,13 @@ void batch_finalize(FinOp op, SolverState *st, const double *partials, const int
                     const int *item_count, const int *nvs, int nsys, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_finalize, dim3(nsys), dim3(256), 0, s, (int)op, st, partials,
                        item_start, item_count, nvs);
+}
+
+void batch_compact(const BatchItem *items, const int *item_start, const int *item_count,
+                   const SolverState *st, int nsys, BatchItem *items_act, int *nact,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_compact, dim3(nsys), dim3(256), 0, s, items, item_start,
+                       item_count, st, nsys, items_act, nact);
 }
 
 void batch_all_done(const SolverState *st, int nsys, int *flag, hipStream_t s) {

@@ -348,7 +348,8 @@ constexpr int kBatchMaxNV = 16;
 struct BatchItem {        // rows [r0, r1) of system `sys`, which has nv RHS
     int32_t sys, nv;
     int64_t r0, r1;
-    int32_t wpl, pad;      // RHS per lane (1 or 2: nv/wpl lanes per row)
+    int32_t wpl;           // RHS per lane (1 or 2: nv/wpl lanes per row)
+    int32_t slot;          // the item's dot-partial slot (its index in the full item list)
 };
 
 struct BatchDev {
@@ -412,8 +413,14 @@ struct BatchSolve {
 int batch_solve_blocks();
 hipError_t batch_solve_launch(const BatchSolve &P, int grid, hipStream_t s);
 
+// nact (nullable, device): the launch walks only the first *nact of the nitems items
 void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nitems, int grid,
-                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s);
+                  const BatchVecs &v, int ci, int last, double *partials, hipStream_t s,
+                  const int *nact = nullptr);
+// items_act / *nact = the items of the systems not done, in item order
+void batch_compact(const BatchItem *items, const int *item_start, const int *item_count,
+                   const SolverState *st, int nsys, BatchItem *items_act, int *nact,
+                   hipStream_t s);
 void batch_finalize(FinOp op, SolverState *st, const double *partials, const int *item_start,
                     const int *item_count, const int *nvs, int nsys, hipStream_t s);
 // flag[0] = every system done, flag[1] = systems not done

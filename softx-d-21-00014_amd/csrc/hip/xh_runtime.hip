@@ -2193,6 +2193,9 @@ struct xh_batch_s {
     int64_t *d_rbase = nullptr, *d_voff = nullptr;
     int *d_nv = nullptr, *d_item_start = nullptr, *d_item_count = nullptr, *d_flag = nullptr;
     BatchItem *d_items = nullptr;
+    // the items of the systems not yet done (rebuilt after every iteration) and their count
+    BatchItem *d_items_act = nullptr;
+    int *d_nact = nullptr;
     SolverState *st = nullptr;
     double *partials = nullptr;
     double *x = nullptr, *b = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr;
@@ -2240,7 +2243,8 @@ void build_items(xh_batch_s *B, unsigned mask, std::vector<BatchItem> &items,
         if (start)
             (*start)[s] = (int)items.size();
         for (int64_t r0 = 0; r0 < B->n[s]; r0 += tile)
-            items.push_back({s, B->nv[s], r0, std::min(B->n[s], r0 + tile), wpl, 0});
+            items.push_back({s, B->nv[s], r0, std::min(B->n[s], r0 + tile), wpl,
+                             (int32_t)items.size()});
         if (count)
             (*count)[s] = (int)items.size() - (*start)[s];
     }
@@ -2249,12 +2253,15 @@ void build_items(xh_batch_s *B, unsigned mask, std::vector<BatchItem> &items,
 int upload_items(xh_batch_s *B) {
     build_items(B, 0x1f, B->items, &B->item_start, &B->item_count);
     (void)hipFree(B->d_items);
+    (void)hipFree(B->d_items_act);
     (void)hipFree(B->partials);
     B->d_items = nullptr;
+    B->d_items_act = nullptr;
     B->partials = nullptr;
     size_t acc = 0;
     int rc;
     if ((rc = dmalloc(&B->d_items, B->items.size(), acc)) ||
+        (rc = dmalloc(&B->d_items_act, B->items.size(), acc)) ||
         (rc = dmalloc(&B->partials, B->items.size() * kBatchPart, acc)))
         return rc;
     XH_HIP(hipMemcpy(B->d_items, B->items.data(), B->items.size() * sizeof(BatchItem),
@@ -2285,10 +2292,13 @@ BatchDev batch_dev(const xh_batch_s *B) {
     return d;
 }
 
-// x = M^-1 b from a zero x: Chebyshev polynomial (smooth() with x_zero) or a copy
+// x = M^-1 b from a zero x: Chebyshev polynomial (smooth() with x_zero) or a copy, over
+// the work list of the systems not yet done
 void batch_precond(xh_batch_s *B, const double *b, double *x) {
     const BatchDev d = batch_dev(B);
     const int nit = (int)B->items.size();
+    const BatchItem *items = B->d_items_act;
+    const int *na = B->d_nact;
     hipStream_t st = B->stream;
     if (B->prm.precond.method == XH_CHEBYSHEV) {
         const int k = B->prm.precond.poly_order;
@@ -2297,7 +2307,7 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
         v.out = k == 1 ? x : B->cu1;
         v.out2 = x;
         v.out3 = B->cr;
-        batch_launch(B_CHEB0Z, d, B->d_items, nit, B->grid, v, k - 1, k == 1, nullptr, st);
+        batch_launch(B_CHEB0Z, d, items, nit, B->grid, v, k - 1, k == 1, nullptr, st, na);
         double *uc = B->cu1, *un = B->cu2;
         for (int i = k - 2; i >= 0; --i) {
             BatchVecs w;
@@ -2305,7 +2315,7 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
             w.o1 = B->cr;
             w.out = un;
             w.out2 = x;
-            batch_launch(B_CHEB, d, B->d_items, nit, B->grid, w, i, i == 0, nullptr, st);
+            batch_launch(B_CHEB, d, items, nit, B->grid, w, i, i == 0, nullptr, st, na);
             std::swap(uc, un);
         }
         return;
@@ -2313,12 +2323,17 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
     BatchVecs v;
     v.b = b;
     v.out = x;
-    batch_launch(B_COPY, d, B->d_items, nit, B->grid, v, 0, 0, nullptr, st);
+    batch_launch(B_COPY, d, items, nit, B->grid, v, 0, 0, nullptr, st, na);
 }
 
+// one iteration over the work list of the systems not yet done (items of a system that
+// finishes during the iteration are skipped by its done flag); the list for the next
+// iteration is rebuilt at the end
 void batch_iteration(xh_batch_s *B) {
     const BatchDev d = batch_dev(B);
     const int nit = (int)B->items.size();
+    const BatchItem *items = B->d_items_act;
+    const int *na = B->d_nact;
     hipStream_t st = B->stream;
     const bool pre = B->prm.solver.method != XH_BICGSTAB;
     double *pp = pre ? B->p_ : B->p, *ss = pre ? B->s_ : B->s;
@@ -2332,20 +2347,20 @@ void batch_iteration(xh_batch_s *B) {
     v1.in = pp;
     v1.o1 = B->r0;
     v1.out = B->v;
-    batch_launch(B_DOT1, d, B->d_items, nit, B->grid, v1, 0, 0, B->partials, st);
+    batch_launch(B_DOT1, d, items, nit, B->grid, v1, 0, 0, B->partials, st, na);
     fin(FIN_ALPHA);
     BatchVecs vs;
     vs.b = B->r;
     vs.o1 = B->v;
     vs.out = B->s;
-    batch_launch(B_S_UPDATE, d, B->d_items, nit, B->grid, vs, 0, 0, nullptr, st);
+    batch_launch(B_S_UPDATE, d, items, nit, B->grid, vs, 0, 0, nullptr, st, na);
     if (pre)
         batch_precond(B, B->s, B->s_);
     BatchVecs v3;
     v3.in = ss;
     v3.o1 = B->s;
     v3.out = B->t;
-    batch_launch(B_DOT3, d, B->d_items, nit, B->grid, v3, 0, 0, B->partials, st);
+    batch_launch(B_DOT3, d, items, nit, B->grid, v3, 0, 0, B->partials, st, na);
     fin(FIN_OMEGA);
     BatchVecs vx;
     vx.x = B->x;
@@ -2355,14 +2370,16 @@ void batch_iteration(xh_batch_s *B) {
     vx.t = B->t;
     vx.o1 = B->r0;
     vx.out = B->r;
-    batch_launch(B_XR_UPDATE, d, B->d_items, nit, B->grid, vx, 0, 0, B->partials, st);
+    batch_launch(B_XR_UPDATE, d, items, nit, B->grid, vx, 0, 0, B->partials, st, na);
     fin(FIN_BETA);
     BatchVecs vp;
     vp.b = B->r;
     vp.o1 = B->v;
     vp.out = B->p;
-    batch_launch(B_P_UPDATE, d, B->d_items, nit, B->grid, vp, 0, 0, nullptr, st);
+    batch_launch(B_P_UPDATE, d, items, nit, B->grid, vp, 0, 0, nullptr, st, na);
     batch_all_done(B->st, B->nsys, B->d_flag, st);
+    batch_compact(B->d_items, B->d_item_start, B->d_item_count, B->st, B->nsys,
+                  B->d_items_act, B->d_nact, st);
 }
 
 void free_batch(xh_batch_s *B) {
@@ -2375,7 +2392,8 @@ void free_batch(xh_batch_s *B) {
     for (void *q : {(void *)B->row, (void *)B->col, (void *)B->val, (void *)B->isd,
                     (void *)B->cheb, (void *)B->d_rbase, (void *)B->d_voff, (void *)B->d_nv,
                     (void *)B->d_item_start, (void *)B->d_item_count, (void *)B->d_flag,
-                    (void *)B->d_items, (void *)B->st, (void *)B->partials, (void *)B->x,
+                    (void *)B->d_items, (void *)B->d_items_act, (void *)B->d_nact,
+                    (void *)B->st, (void *)B->partials, (void *)B->x,
                     (void *)B->b, (void *)B->r0, (void *)B->r, (void *)B->p, (void *)B->v,
                     (void *)B->s, (void *)B->t, (void *)B->p_, (void *)B->s_, (void *)B->cr,
                     (void *)B->cu1, (void *)B->cu2, (void *)B->d_t0, (void *)B->d_bar})
@@ -2491,6 +2509,7 @@ extern "C" int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nv
         (rc = dmalloc(&B->d_rbase, nsys, acc)) || (rc = dmalloc(&B->d_voff, nsys, acc)) ||
         (rc = dmalloc(&B->d_nv, nsys, acc)) || (rc = dmalloc(&B->d_item_start, nsys, acc)) ||
         (rc = dmalloc(&B->d_item_count, nsys, acc)) || (rc = dmalloc(&B->d_flag, 2, acc)) ||
+        (rc = dmalloc(&B->d_nact, 1, acc)) ||
         (rc = dmalloc(&B->st, nsys, acc)) || (rc = dmalloc(&B->d_t0, 1, acc)) ||
         (rc = dmalloc(&B->d_bar, 1, acc)))
         return bail(rc);
@@ -2623,6 +2642,8 @@ extern "C" int xh_batch_solve(xh_batch B) {
     batch_launch(B_RESID0, d, B->d_items, (int)B->items.size(), B->grid, v, 0, 0, B->partials, st);
     batch_finalize(FIN_INIT, B->st, B->partials, B->d_item_start, B->d_item_count, B->d_nv,
                    nsys, st);
+    batch_compact(B->d_items, B->d_item_start, B->d_item_count, B->st, nsys, B->d_items_act,
+                  B->d_nact, st);
     int rc = check_launch();
     if (rc)
         return rc;
