@@ -41,10 +41,11 @@ __device__ __forceinline__ void item_partials(const double (&part)[3][W], int nd
     __syncthreads();
 }
 
-// One work item of one batched operation with W right-hand sides per lane (nv/W lanes
-// per row, kBlock*W/nv rows per pass).  Each RHS gets the arithmetic of the single-system
-// kernels; W changes only which lane holds it and so the grouping of the dot partials.
-template <int OP, int W>
+// One work item of one batched operation with W right-hand sides per lane (L = nv/W
+// lanes per row, kBlock/L rows per pass).  Each RHS gets the arithmetic of the
+// single-system kernels; W changes only which lane holds it and so the grouping of the
+// dot partials.
+template <int OP, int W, int L>
 __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w,
                                            const BatchVecs &v, int ci, int last,
                                            double *__restrict__ partials) {
@@ -53,7 +54,8 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                                                                              : 0;
     const int tid = threadIdx.x;
     const SolverState *st = d.st + w.sys;
-    const int nv = w.nv, lanes = nv / W, rpp = kBlock / lanes;
+    constexpr int lanes = L, rpp = kBlock / L;
+    const int nv = w.nv;
     const int tr = tid / lanes, k0 = (tid % lanes) * W;
     const int64_t gb = d.rbase[w.sys];
     const size_t vo = (size_t)d.voff[w.sys];
@@ -91,38 +93,71 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
 #pragma unroll
                 for (int u = 0; u < W; ++u)
                     acc[u] = 0.0;
-                // entries in batches of EB: columns and values of the batch, then its x
-                // rows, all in flight together; slots past the row end re-read the last
-                // entry (in bounds) and are not added, so the sum keeps the stored order
-                constexpr int EB = 4;
                 const uint32_t j1 = d.row[g + 1];
-                for (uint32_t j = d.row[g]; j < j1; j += EB) {
-                    uint32_t c[EB];
-                    double a[EB], xv[EB][W];
-#pragma unroll
-                    for (int q = 0; q < EB; ++q) {
-                        const uint32_t jj = j + q < j1 ? j + q : j1 - 1;
-                        c[q] = d.col[jj];
-                        a[q] = d.val[jj];
+                // x row of entry column c: W consecutive RHS (16-B aligned for W = 2:
+                // system vectors start on 256 B, k0 and nv even)
+                auto ldx = [&](uint32_t c, double (&xq)[W]) __attribute__((always_inline)) {
+                    const double *xp = xs + (size_t)c * nv + k0;
+                    if constexpr (W == 2) {
+                        const double2 t2 = *(const double2 *)xp;
+                        xq[0] = t2.x;
+                        xq[1] = t2.y;
+                    } else {
+                        xq[0] = xp[0];
                     }
+                };
+                if constexpr (L == 1) {
+                    // entries in batches of EB: columns and values of the batch, then its x
+                    // rows, all in flight together; slots past the row end re-read the last
+                    // entry (in bounds) and are not added, so the sum keeps the stored order
+                    constexpr int EB = 4;
+                    for (uint32_t j = d.row[g]; j < j1; j += EB) {
+                        uint32_t c[EB];
+                        double a[EB], xv[EB][W];
 #pragma unroll
-                    for (int q = 0; q < EB; ++q) {
-                        const double *xp = xs + (size_t)c[q] * nv + k0;
-                        if constexpr (W == 2) {
-                            // 16-B aligned: system vectors start on 256 B, k0 and nv even
-                            const double2 t2 = *(const double2 *)xp;
-                            xv[q][0] = t2.x;
-                            xv[q][1] = t2.y;
-                        } else {
-                            xv[q][0] = xp[0];
+                        for (int q = 0; q < EB; ++q) {
+                            const uint32_t jj = j + q < j1 ? j + q : j1 - 1;
+                            c[q] = d.col[jj];
+                            a[q] = d.val[jj];
                         }
+#pragma unroll
+                        for (int q = 0; q < EB; ++q)
+                            ldx(c[q], xv[q]);
+#pragma unroll
+                        for (int q = 0; q < EB; ++q)
+                            if (j + q < j1)
+#pragma unroll
+                                for (int u = 0; u < W; ++u)
+                                    acc[u] = acc[u] + xv[q][u] * a[q];
                     }
+                } else {
+                    // the L lanes of the row load L consecutive entries, one each, and
+                    // broadcast column and value across the row's lanes (DPP); then the L x
+                    // rows are loaded together and added in stored order (slots past the
+                    // row end re-read the last entry and are not added).  The lanes of a
+                    // row are active together, as the broadcast needs.
+                    const uint32_t lr = (uint32_t)(tid % L);
+                    for (uint32_t j = d.row[g]; j < j1; j += L) {
+                        const uint32_t jm = j + lr < j1 ? j + lr : j1 - 1;
+                        const uint32_t cm = d.col[jm];
+                        const double am = d.val[jm];
+                        uint32_t c[L];
+                        double a[L], xv[L][W];
+                        static_for<0, L>([&](auto qc) {
+                            constexpr int q = decltype(qc)::value;
+                            c[q] = (uint32_t)bcast<L, q>((int)cm);
+                            a[q] = bcast<L, q>(am);
+                        });
 #pragma unroll
-                    for (int q = 0; q < EB; ++q)
-                        if (j + q < j1)
+                        for (int q = 0; q < L; ++q)
+                            ldx(c[q], xv[q]);
 #pragma unroll
-                            for (int u = 0; u < W; ++u)
-                                acc[u] = acc[u] + xv[q][u] * a[q];
+                        for (int q = 0; q < L; ++q)
+                            if (j + q < j1)
+#pragma unroll
+                                for (int u = 0; u < W; ++u)
+                                    acc[u] = acc[u] + xv[q][u] * a[q];
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < W; ++u) {
@@ -202,10 +237,36 @@ __device__ __forceinline__ void batch_phase(const BatchDev &d,
         const BatchItem w = items[it];
         if (d.st[w.sys].done)
             continue; // uniform over the workgroup: one item per workgroup at a time
-        if (w.wpl == 2)
-            batch_item<OP, 2>(d, w, v, ci, last, partials);
-        else
-            batch_item<OP, 1>(d, w, v, ci, last, partials);
+        // lanes per row L = nv / (RHS per lane), uniform over the item
+        switch (w.nv / w.wpl) {
+        case 1:
+            if (w.wpl == 2)
+                batch_item<OP, 2, 1>(d, w, v, ci, last, partials);
+            else
+                batch_item<OP, 1, 1>(d, w, v, ci, last, partials);
+            break;
+        case 2:
+            if (w.wpl == 2)
+                batch_item<OP, 2, 2>(d, w, v, ci, last, partials);
+            else
+                batch_item<OP, 1, 2>(d, w, v, ci, last, partials);
+            break;
+        case 4:
+            if (w.wpl == 2)
+                batch_item<OP, 2, 4>(d, w, v, ci, last, partials);
+            else
+                batch_item<OP, 1, 4>(d, w, v, ci, last, partials);
+            break;
+        case 8:
+            if (w.wpl == 2)
+                batch_item<OP, 2, 8>(d, w, v, ci, last, partials);
+            else
+                batch_item<OP, 1, 8>(d, w, v, ci, last, partials);
+            break;
+        default:
+            batch_item<OP, 1, 16>(d, w, v, ci, last, partials);
+            break;
+        }
     }
 }
 

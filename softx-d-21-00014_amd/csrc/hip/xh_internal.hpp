@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 namespace xh {
 
@@ -447,5 +448,57 @@ void copy_vec(int64_t count, int prec, const void *src, void *dst, const int *do
 // x = u + x
 void add_vec(int64_t count, int prec, const void *u, void *x, const int *done, hipStream_t s);
 void fill_zero(int64_t count, int prec, void *dst, const int *done, hipStream_t s);
+
+// ---- lane helpers shared by the kernels ----
+
+// Broadcast of lane S of each L-lane group (L consecutive lanes, L | 64) to the whole
+// group, S a compile-time constant.  DPP (a VALU operand modifier, no LDS traffic):
+// quad_perm for L = 2, 4; row_newbcast (gfx90a+: every lane of a 16-lane row reads
+// lane S of that row) for L = 16, and two of them merged for L = 8.  L = 32 goes
+// through ds_bpermute.  The lanes of a group share one row, so a group is active or
+// inactive as a whole and every source lane read by an active lane is active: the DPP
+// moves need no "old" operand.
+template <int L, int S>
+__device__ __forceinline__ int bcast_i32(int v) {
+    static_assert(S >= 0 && S < L, "source lane outside the group");
+    if constexpr (L == 2) {
+        constexpr int q = S | (S << 2) | ((2 + S) << 4) | ((2 + S) << 6);
+        return __builtin_amdgcn_mov_dpp(v, q, 0xF, 0xF, false);
+    } else if constexpr (L == 4) {
+        constexpr int q = S | (S << 2) | (S << 4) | (S << 6);
+        return __builtin_amdgcn_mov_dpp(v, q, 0xF, 0xF, false);
+    } else if constexpr (L == 16) {
+        return __builtin_amdgcn_mov_dpp(v, 0x150 + S, 0xF, 0xF, false);
+    } else if constexpr (L == 8) {
+        const int lo = __builtin_amdgcn_mov_dpp(v, 0x150 + S, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp(v, 0x158 + S, 0xF, 0xF, false);
+        return (threadIdx.x & 8) ? hi : lo;
+    } else {
+        const int lane = threadIdx.x & 63;
+        return __shfl(v, lane - (lane % L) + S, 64);
+    }
+}
+
+template <int L, int S, typename T>
+__device__ __forceinline__ T bcast(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, bcast_i32<L, S>(__builtin_bit_cast(int, v)));
+    } else {
+        static_assert(sizeof(T) == 8, "4- or 8-byte values");
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const int lo = bcast_i32<L, S>((int)(unsigned)u);
+        const int hi = bcast_i32<L, S>((int)(unsigned)(u >> 32));
+        return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+
+// compile-time loop: f(std::integral_constant<int, 0>), ..., f(<N-1>)
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>());
+        static_for<I + 1, N>(f);
+    }
+}
 
 } // namespace xh
