@@ -121,6 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_vec(int64_t n, int64_t chunk, double
     const double d2 = (OP == V_RB_UPDATE) ? st->nomega[k] * be : 0.0;
     const double om = (OP == V_RB_UPDATE_MERGED) ? st->omega[k] : 0.0;
     const double bo = (OP == V_RB_UPDATE_MERGED) ? be * om : 0.0;
+    // unrolled so that the loads of several rows are in flight together; each thread
+    // still visits its rows, and adds its dot terms, in row order
+#pragma unroll 4
     for (int64_t i = start + tr; i < end; i += RPB) {
         const size_t o = (size_t)i * NV + k;
         if constexpr (OP == V_S_UPDATE) {
