@@ -171,6 +171,9 @@ struct Tuning {
     // cube at 128^3 only matches the row-parallel kernel at nv = 32 (slower at other nv
     // and in fp32); the 27-point operator at 96^3 gains 10-20 %
     int grid_enable = 2;
+    // 1: a Chebyshev smoother from a zero x runs its first two steps as one SpMV
+    // (OP_CHEBZ) where OP_CHEB would run row-parallel
+    int chebz_enable = 1;
     int grid_items = 2048;          // target work items (tile x segment) per launch
     int grid_min_seg = 32;          // shortest x segment of a work item
     // 1: matrices with a complete 7-point stencil view run the register-window line
@@ -250,6 +253,10 @@ enum SpmvOp : int {
     OP_RESID0 = 9,   // y = b - A x, b and y f64, written both f64 and group1-fused
     OP_RDOT4 = 10,   // t = A t_ ; r^ = r - alpha v (r: out2, v: b, alpha: rr) ; partials
                      // (t, r^), (t, t), (t, r0), (r^, r^) (r0: o1)  (RBiCGStab steps 11-13)
+    OP_CHEBZ = 11,   // the first two Chebyshev steps from a zero-flagged x in one launch:
+                     // the operand is b, each gathered entry u[c] = (c2*(b[c]*isd[c]))*isd[c]
+                     // (the zero-x first step), r = b*isd written unless last, then as
+                     // OP_CHEB: t1 = c*r + (A u)*isd ; u' = t1*isd [or x = t1*isd when last]
 };
 
 struct SpmvArgs {
@@ -261,6 +268,7 @@ struct SpmvArgs {
     void *out3 = nullptr;       // RESID0: r
     void *xupd = nullptr;       // CHEB*: x to update when `last`
     double c = 0.0;             // CHEB*: polynomial coefficient (cast to the matrix prec)
+    double c2 = 0.0;            // CHEBZ: coefficient of the zero-x first step
     int last = 0;               // CHEB*: this step ends the polynomial (updates x)
     int x_zero = 0;             // CHEB*: x is zero-flagged (x = t1 instead of x + t1)
     int out_f32 = 0;            // RESID: store y as float (mixed-precision restriction)
@@ -273,6 +281,8 @@ struct SpmvArgs {
 // returns the number of partial blocks written (DOT ops) or 0
 int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
          hipStream_t s);
+// whether op on A runs on the row-parallel kernel (neither the stencil nor the grid path)
+bool spmv_row_parallel(const Mat &A, SpmvOp op, int nv);
 
 // ---- vector kernels, finalize, smoothers (xh_vec.hip) ----
 enum VecOp : int {

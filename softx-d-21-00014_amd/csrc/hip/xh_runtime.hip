@@ -719,7 +719,23 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
     case XH_CHEBYSHEV: {
         const int k = role.poly_order;
         void *uc = L.su1, *un = L.su2;
-        if (x_zero) {
+        int first = k - 2; // the first coefficient index left for the OP_CHEB loop
+        if (x_zero && k >= 2 && g_tune.chebz_enable && L.A.idx != IDX_U8 &&
+            spmv_row_parallel(L.A, OP_CHEB, nv)) {
+            // the zero-x first step and the next one in one launch: the SpMV gathers b and
+            // forms u = (c*(b*isd))*isd per entry, bit-identical to the two launches
+            SpmvArgs a = args(S);
+            a.b = b;
+            a.isd = L.A.inv_sqrt_diag;
+            a.out2 = L.sr;
+            a.c = cheb[k - 2];
+            a.c2 = cheb[k - 1];
+            a.last = k == 2;
+            a.xupd = x;
+            a.x_zero = 1;
+            spmv(L.A, nv, b, uc, OP_CHEBZ, a, st);
+            first = k - 3;
+        } else if (x_zero) {
             cheb_init_zero(L.n, nv, L.prec, b, L.A.inv_sqrt_diag, cheb[k - 1], L.sr,
                            k == 1 ? x : uc, k == 1, S->dflag, st);
         } else {
@@ -737,7 +753,7 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
             if (k == 1)
                 add_vec(L.n * nv, L.prec, uc, x, S->dflag, st);
         }
-        for (int i = k - 2; i >= 0; --i) {
+        for (int i = first; i >= 0; --i) {
             SpmvArgs a = args(S);
             a.isd = L.A.inv_sqrt_diag;
             a.r = L.sr;
@@ -1912,6 +1928,8 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.fold_enable = (int)value;
     else if (k == "spmv_fold_min_bytes" && value >= 0)
         g_tune.fold_min_bytes = value;
+    else if (k == "cheb_fuse_zero" && (value == 0 || value == 1))
+        g_tune.chebz_enable = (int)value;
     else if (k == "spmv_grid_enable" && value >= 0 && value <= 2)
         g_tune.grid_enable = (int)value;
     else if (k == "spmv_st7_enable" && (value == 0 || value == 1))

@@ -80,11 +80,16 @@ struct Chunk {
     static constexpr int CH = L >= 2 ? LL * E : BW;     // entries per chunk: 8 or 16
     uint32_t c[E]; // raw column index (panel-relative for u16)
     VT v[E];
+    VT d[E];       // SCALE loads: the scale of the entry's column (sd[base + c])
 };
 
-template <typename VT, typename IT, int NV, int W>
+// SCALE: also the column's scale sd[base + c] of each loaded entry (a load that depends on
+// the column's), broadcast with the column and value
+template <typename VT, typename IT, int NV, int W, bool SCALE = false>
 __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
-                                           const VT *__restrict__ val, Chunk<VT, NV, W> &ch) {
+                                           const VT *__restrict__ val, Chunk<VT, NV, W> &ch,
+                                           uint32_t base = 0,
+                                           const VT *__restrict__ sd = nullptr) {
     // Entries past the row end repeat the row's last entry (its x row exists), so the
     // x loads of a batch need no select; their products are never added.
     using C = Chunk<VT, NV, W>;
@@ -97,6 +102,11 @@ __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__r
         ch.c[q] = live ? (uint32_t)col[jc] : 0u;
         ch.v[q] = live ? val[jc] : VT(0);
     }
+    if constexpr (SCALE) {
+#pragma unroll
+        for (int q = 0; q < C::E; ++q)
+            ch.d[q] = live ? sd[base + ch.c[q]] : VT(0);
+    }
 }
 
 // acc[w] + sum x[base + c][k0 + w] * v over the entries [j, min(j + CH, e)) of `ch`.
@@ -105,14 +115,25 @@ __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__r
 // slot past the row end issues none (1-entry rows: 303 -> 217 us at 128^3, nv = 32);
 // the accumulation of a slot past the row end is skipped the same way.
 // SKIP: entries whose column lies in [slo, slo + sn) are skipped the same way (neither
-// loaded nor added), which keeps the order of the others.
-template <typename VT, int NV, int W, bool SKIP = false>
+// loaded nor added), which keeps the order of the others.  SCALE: each gathered value
+// x[c] enters as (cz*(x[c]*d))*d with d the column's scale from the chunk (the zero-x
+// first Chebyshev step of OP_CHEBZ, rounded as that step rounds it).
+template <typename VT, int NV, int W, bool SKIP = false, bool SCALE = false>
 __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chunk<VT, NV, W> &ch,
                                               uint32_t base, const VT *__restrict__ x, int k0,
-                                              VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0) {
+                                              VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0,
+                                              VT cz = VT(0)) {
     using C = Chunk<VT, NV, W>;
     constexpr int BW = C::BW, L = C::L;
     auto kept = [&](uint32_t cabs) { return !SKIP || cabs - slo >= sn; };
+    auto scale = [&](Pack<VT, W> &xv, VT dq) __attribute__((always_inline)) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            VT t = xv.v[w] * dq;
+            t = cz * t;
+            xv.v[w] = t * dq;
+        }
+    };
     if constexpr (L >= 2) {
         bool more = true; // uniform within the row's lane group
         static_for<0, C::CH / BW>([&](auto bc) {
@@ -138,6 +159,8 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
                 constexpr int t = b0 + q;
                 if (j + t < e && kept(base + c[q])) {
                     const VT vq = bcast<L, t % C::LL>(ch.v[t / C::LL]);
+                    if constexpr (SCALE)
+                        scale(xv[q], bcast<L, t % C::LL>(ch.d[t / C::LL]));
 #pragma unroll
                     for (int w = 0; w < W; ++w)
                         acc[w] = acc[w] + xv[q].v[w] * vq;
@@ -153,21 +176,25 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
             xv[q] = ld<VT, W>(x + (size_t)(base + ch.c[q]) * NV + k0);
 #pragma unroll
         for (int q = 0; q < BW; ++q)
-            if (j + q < e && kept(base + ch.c[q]))
+            if (j + q < e && kept(base + ch.c[q])) {
+                if constexpr (SCALE)
+                    scale(xv[q], ch.d[q]);
 #pragma unroll
                 for (int w = 0; w < W; ++w)
                     acc[w] = acc[w] + xv[q].v[w] * ch.v[q];
+            }
     }
 }
 
 // the whole row [j, e), its first chunk already in `first` (SKIP: without the columns
 // in [slo, slo + sn))
-template <typename VT, typename IT, int NV, int W, bool SKIP = false>
+template <typename VT, typename IT, int NV, int W, bool SKIP = false, bool SCALE = false>
 __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
                                         const Chunk<VT, NV, W> &first,
                                         const IT *__restrict__ col, const VT *__restrict__ val,
                                         uint32_t base, const VT *__restrict__ x, int k0,
-                                        VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0) {
+                                        VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0,
+                                        const VT *__restrict__ sd = nullptr, VT cz = VT(0)) {
     using C = Chunk<VT, NV, W>;
     // a longer row loads its next chunk before the x loads of the current one, so the
     // chunk's latency overlaps them
@@ -176,8 +203,8 @@ __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
         const bool more = j + C::CH < e; // the same for all lanes of the row
         C nxt;
         if (more)
-            load_chunk<VT, IT, NV, W>(j + C::CH, e, col, val, nxt);
-        consume_chunk<VT, NV, W, SKIP>(j, e, cur, base, x, k0, acc, slo, sn);
+            load_chunk<VT, IT, NV, W, SCALE>(j + C::CH, e, col, val, nxt, base, sd);
+        consume_chunk<VT, NV, W, SKIP, SCALE>(j, e, cur, base, x, k0, acc, slo, sn, cz);
         if (!more)
             break;
         j += C::CH;
@@ -267,9 +294,9 @@ __device__ __forceinline__ void epi_load(int64_t i, size_t o, size_t oy, int k0,
 #pragma unroll
         for (int w = 0; w < W; ++w)
             in.al[w] = rr[k0 + w];
-    } else if constexpr (OP == OP_CHEB0 || OP == OP_CHEB) {
+    } else if constexpr (OP == OP_CHEB0 || OP == OP_CHEB || OP == OP_CHEBZ) {
         in.di = isd[i];
-        if constexpr (OP == OP_CHEB0)
+        if constexpr (OP == OP_CHEB0 || OP == OP_CHEBZ)
             in.a = ld<VT, W>(b + o);
         else
             in.a = ld<VT, W>(rr + o);
@@ -401,6 +428,21 @@ __device__ __forceinline__ void row_epilogue(
                 t1.v[w] = t1.v[w] + in.b.v[w];
         }
         st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
+    } else if constexpr (OP == OP_CHEBZ) {
+        // r = b*isd (the zero-x first step's residual, kept for the later steps unless
+        // this step is the last) ; t2 = (A u)*isd ; t1 = c*r + t2 ; u' or x = t1*isd
+        const VT di = in.di;
+        P rv, t1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            rv.v[w] = in.a.v[w] * di;
+            const VT t2 = acc[w] * di;
+            t1.v[w] = c * rv.v[w] + t2;
+            t1.v[w] = t1.v[w] * di;
+        }
+        if (!last)
+            st<VT, W>(out2 + o, rv);
+        st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
     } else if constexpr (OP == OP_PROLONG32) {
         // VT = float: temp32 = P xH ; x64 = (double)temp32 + x64
         double *xd = (double *)yv;
@@ -420,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const VT *__restrict__ o1, const VT *__restrict__ isd, const VT *__restrict__ rr,
     VT *__restrict__ out2, VT *__restrict__ out3, void *__restrict__ xupdv, VT c, int last,
     int x_zero, int out_f32, double *__restrict__ partials, const int *__restrict__ done,
-    int64_t blk_rows, const uint32_t *__restrict__ rind) {
+    int64_t blk_rows, const uint32_t *__restrict__ rind, VT c2) {
     constexpr int ND = NDots<OP>::value;
     constexpr int W = Width<VT, NV>::value;
     constexpr int L = NV / W;        // lanes per row
@@ -457,8 +499,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         ne = row[i + RPB + 1];
         nbs = panel_base<IT>(pbase, shift, i + RPB);
     }
+    constexpr bool SC = OP == OP_CHEBZ; // chunks carry their columns' isd
     Chunk<VT, NV, W> ch;
-    load_chunk<VT, IT, NV, W>(jb, je, col, val, ch);
+    load_chunk<VT, IT, NV, W, SC>(jb, je, col, val, ch, bs, isd);
     // Drain the prologue loads once per tile: the row loop's head then merges only its
     // own back edge, where the last y store may stay in flight.
     __builtin_amdgcn_s_waitcnt(0);
@@ -466,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         const size_t o = (size_t)i * NV + k0;
         const uint32_t base = bs;
         Chunk<VT, NV, W> nch;
-        load_chunk<VT, IT, NV, W>(nb, ne, col, val, nch);
+        load_chunk<VT, IT, NV, W, SC>(nb, ne, col, val, nch, nbs, isd);
         uint32_t nnb = 0, nne = 0, nnbs = 0;
         if (i + 2 * RPB < end) {
             nnb = row[i + 2 * RPB];
@@ -493,6 +536,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             const uint32_t lo = (uint32_t)((i / blk_rows) * blk_rows);
             row_acc<VT, IT, NV, W, true>(jb, je, ch, col, val, base, x, k0, acc, lo,
                                          (uint32_t)blk_rows);
+        } else if constexpr (OP == OP_CHEBZ) {
+            // the operand is b; each gathered entry enters as the zero-x first step's u
+            row_acc<VT, IT, NV, W, false, true>(jb, je, ch, col, val, base, x, k0, acc, 0u, 0u,
+                                                isd, c2);
         } else {
             row_acc<VT, IT, NV, W>(jb, je, ch, col, val, base, x, k0, acc);
         }
@@ -1107,7 +1154,8 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int W = Width<VT, NV>::value;
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? g_tune.dot_groups : (a.grid_cap > 0 ? a.grid_cap : 16384);
-    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 && NV >= 8) {
+    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
+                  OP != OP_CHEBZ && NV >= 8) {
         if (A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr) {
             constexpr int G = 64 / (NV / W);
             const StLaunch S =
@@ -1123,7 +1171,8 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     // the off-block residual of hybrid GS stays on the row path: decoding each entry's
     // column to skip the row's own block made the grid sweep slower (466 against 369 us on
     // the 96^3 27-point operator)
-    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32) {
+    if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
+                  OP != OP_CHEBZ) {
         if (A.gcode != nullptr && grid_on(A) && A.row_ind == nullptr) {
             const GridLaunch G = grid_plan(A.gx, A.gy, A.gz, RPB, cap);
             hipLaunchKernelGGL((k_spmv_grid<VT, NV, OP>), dim3(G.grid), dim3(kBlock), 0, s,
@@ -1141,7 +1190,7 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
                        L.sc, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,
                        (const VT *)a.isd, (const VT *)a.r, (VT *)a.out2, (VT *)a.out3, a.xupd,
                        (VT)a.c, a.last, a.x_zero, a.out_f32, a.partials, a.done, a.blk_rows,
-                       A.row_ind);
+                       A.row_ind, (VT)a.c2);
     return ND > 0 ? L.grid : 0;
 }
 
@@ -1168,6 +1217,8 @@ int by_op(const Mat &A, const void *x, void *y, SpmvOp op, const SpmvArgs &a, hi
         return launch_t<VT, IT, NV, OP_CHEB0>(A, x, y, a, s);
     case OP_CHEB:
         return launch_t<VT, IT, NV, OP_CHEB>(A, x, y, a, s);
+    case OP_CHEBZ:
+        return launch_t<VT, IT, NV, OP_CHEBZ>(A, x, y, a, s);
     case OP_OFFBLOCK:
         return launch_t<VT, IT, NV, OP_OFFBLOCK>(A, x, y, a, s);
     default:
@@ -1223,8 +1274,8 @@ namespace {
 const char *op_name(SpmvOp op) {
     static const char *names[] = {"OP_SET",  "OP_ADD",       "OP_RESID",    "OP_DOT1",
                                   "OP_DOT3", "OP_CHEB0",     "OP_CHEB",     "OP_PROLONG32",
-                                  "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4"};
-    return (op >= 0 && op <= OP_RDOT4) ? names[op] : "OP_?";
+                                  "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4", "OP_CHEBZ"};
+    return (op >= 0 && op <= OP_CHEBZ) ? names[op] : "OP_?";
 }
 
 int op_ndots(SpmvOp op) {
@@ -1233,12 +1284,13 @@ int op_ndots(SpmvOp op) {
 
 // the launch_t conditions for the 7-point stencil and the structured-grid kernels
 bool takes_st7(const Mat &A, SpmvOp op, int nv) {
-    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && nv >= 8 &&
+    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && op != OP_CHEBZ &&
+           nv >= 8 &&
            A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr;
 }
 bool takes_grid(const Mat &A, SpmvOp op) {
-    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && A.gcode != nullptr &&
-           grid_on(A) && A.row_ind == nullptr;
+    return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && op != OP_CHEBZ &&
+           A.gcode != nullptr && grid_on(A) && A.row_ind == nullptr;
 }
 
 // Algorithmic HBM bytes of one SpMV launch: the matrix streams once (values, column
@@ -1287,6 +1339,10 @@ double spmv_bytes(const Mat &A, int nv, SpmvOp op, const SpmvArgs &a, bool grid,
         // r, isd ; last: x written (and read unless zero-flagged), else u' written
         vec = V + n * sv + (a.last ? (a.x_zero ? V : 2.0 * V) : V);
         break;
+    case OP_CHEBZ:
+        // b is the operand (counted with x) ; isd ; last: x written, else u' and r written
+        vec = n * sv + (a.last ? V : 2.0 * V);
+        break;
     case OP_PROLONG32:
         vec = n * nv * 8.0 * (a.x_zero ? 1.0 : 2.0); // fp64 x written (and read)
         break;
@@ -1315,6 +1371,11 @@ int spmv_dispatch(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const
 const char *prec_name(int prec) { return prec == F64 ? "double" : "float"; }
 const char *idx_name(int idx) {
     return idx == IDX_U32 ? "u32" : idx == IDX_U16 ? "u16" : "u8";
+}
+
+// whether op on A runs on the row-parallel kernel (neither the stencil nor the grid path)
+bool spmv_row_parallel(const Mat &A, SpmvOp op, int nv) {
+    return !takes_st7(A, op, nv) && !takes_grid(A, op);
 }
 
 int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs &a,
