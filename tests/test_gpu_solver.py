@@ -159,6 +159,36 @@ def test_preconditioner_bit_exact(gpu, O, smoother, k, mg_prec):
 
 
 @pytest.mark.parametrize("mg_prec", [0, 2])
+@pytest.mark.parametrize("k", [2, 4])
+def test_fused_zero_x_chebyshev_step_is_identical(gpu, k, mg_prec):
+    # the pre-smoother's zero-x first step fused into the next SpMV (OP_CHEBZ) against the
+    # two launches, every nv; the launches of a solve show which ran
+    X = gpu
+    N = 12
+    levels = levels_for(X, N)
+    prm = X.default_params(method="MergedPBiCGStab", smoother="Chebyshev", poly_order=k,
+                           sweeps=1, mg_prec=mg_prec)
+    try:
+        for nv in (1, 2, 4, 8, 16, 32, 64):
+            print(f"case k {k} mg_prec {mg_prec} nv {nv}", flush=True)
+            b = X.rhs_uniform(nv, N ** 3, nv)
+            out, names = [], []
+            for fuse in (0, 1):
+                X.set_option("cheb_fuse_zero", fuse)
+                S = X.Solver(X.Hier(levels), prm, nv)
+                db, dx = X.DeviceArray.from_host(b), X.DeviceArray(N ** 3 * nv)
+                S.precond_apply(db, dx)
+                out.append(dx.to_host())
+                dx.zero()
+                names.append({q["kernel"] for q in S.profile(db, dx, 1)})
+            assert np.array_equal(out[0], out[1]), nv
+            assert not any("OP_CHEBZ" in q for q in names[0])
+            assert any("OP_CHEBZ" in q for q in names[1]), names[1]
+    finally:
+        X.set_option("cheb_fuse_zero", 1)
+
+
+@pytest.mark.parametrize("mg_prec", [0, 2])
 def test_hsgs_core_path_bit_exact(gpu, O, mg_prec):
     # blocks small enough for the staged core-block sweep in both precisions (32-row
     # blocks: split_range gives 40 rows); the launches of a solve show that kernel ran
