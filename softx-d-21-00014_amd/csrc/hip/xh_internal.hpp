@@ -171,6 +171,7 @@ struct Tuning {
     // cube at 128^3 only matches the row-parallel kernel at nv = 32 (slower at other nv
     // and in fp32); the 27-point operator at 96^3 gains 10-20 %
     int grid_enable = 2;
+    int grid_dot_groups = 2048;     // work items of a grid-path launch that writes partials
     // 1: a Chebyshev smoother from a zero x runs its first two steps as one SpMV
     // (OP_CHEBZ) where OP_CHEB would run row-parallel
     int chebz_enable = 1;
