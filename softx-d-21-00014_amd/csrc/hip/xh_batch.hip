@@ -79,15 +79,18 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
         } else if constexpr (OP == B_P_UPDATE) {
             c1[u] = st->pcoef[k];
             c2[u] = st->beta[k];
-        } else if constexpr (OP == B_CHEB0Z || OP == B_CHEB) {
+        } else if constexpr (OP == B_CHEB0Z || OP == B_CHEB || OP == B_CHEBZ) {
             c1[u] = d.cheb[(size_t)w.sys * kChebMax + ci];
         }
     }
+    // B_CHEBZ: the zero-x first step's coefficient, applied to every gathered entry
+    const double cz = OP == B_CHEBZ ? d.cheb[(size_t)w.sys * kChebMax + ci + 1] : 0.0;
     if (tr < rpp)
         for (int64_t i = w.r0 + tr; i < w.r1; i += rpp) {
             const size_t ob = vo + (size_t)i * nv + k0;
             const int64_t g = gb + i;
-            if constexpr (OP == B_RESID0 || OP == B_DOT1 || OP == B_DOT3 || OP == B_CHEB) {
+            if constexpr (OP == B_RESID0 || OP == B_DOT1 || OP == B_DOT3 || OP == B_CHEB ||
+                          OP == B_CHEBZ) {
                 const double *xs = v.in + vo;
                 double acc[W];
 #pragma unroll
@@ -106,6 +109,17 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         xq[0] = xp[0];
                     }
                 };
+                // B_CHEBZ: a gathered entry enters as the zero-x first step's value
+                // (cz*(b*isd_c))*isd_c, rounded as that step rounds it
+                auto zstep = [&](double (&xq)[W], double dc) __attribute__((always_inline)) {
+                    if constexpr (OP == B_CHEBZ)
+#pragma unroll
+                        for (int u = 0; u < W; ++u) {
+                            double t = xq[u] * dc;
+                            t = cz * t;
+                            xq[u] = t * dc;
+                        }
+                };
                 if constexpr (L == 1) {
                     // entries in batches of EB: columns and values of the batch, then its x
                     // rows, all in flight together; slots past the row end re-read the last
@@ -113,7 +127,7 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                     constexpr int EB = 4;
                     for (uint32_t j = d.row[g]; j < j1; j += EB) {
                         uint32_t c[EB];
-                        double a[EB], xv[EB][W];
+                        double a[EB], xv[EB][W], dz[EB];
 #pragma unroll
                         for (int q = 0; q < EB; ++q) {
                             const uint32_t jj = j + q < j1 ? j + q : j1 - 1;
@@ -121,14 +135,19 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                             a[q] = d.val[jj];
                         }
 #pragma unroll
-                        for (int q = 0; q < EB; ++q)
+                        for (int q = 0; q < EB; ++q) {
                             ldx(c[q], xv[q]);
+                            if constexpr (OP == B_CHEBZ)
+                                dz[q] = d.isd[gb + c[q]];
+                        }
 #pragma unroll
                         for (int q = 0; q < EB; ++q)
-                            if (j + q < j1)
+                            if (j + q < j1) {
+                                zstep(xv[q], dz[q]);
 #pragma unroll
                                 for (int u = 0; u < W; ++u)
                                     acc[u] = acc[u] + xv[q][u] * a[q];
+                            }
                     }
                 } else {
                     // the L lanes of the row load L consecutive entries, one each, and
@@ -141,22 +160,28 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         const uint32_t jm = j + lr < j1 ? j + lr : j1 - 1;
                         const uint32_t cm = d.col[jm];
                         const double am = d.val[jm];
+                        // B_CHEBZ: the entry's column scale, loaded by the same lane
+                        const double dm = OP == B_CHEBZ ? d.isd[gb + cm] : 0.0;
                         uint32_t c[L];
-                        double a[L], xv[L][W];
+                        double a[L], xv[L][W], dz[L];
                         static_for<0, L>([&](auto qc) {
                             constexpr int q = decltype(qc)::value;
                             c[q] = (uint32_t)bcast<L, q>((int)cm);
                             a[q] = bcast<L, q>(am);
+                            if constexpr (OP == B_CHEBZ)
+                                dz[q] = bcast<L, q>(dm);
                         });
 #pragma unroll
                         for (int q = 0; q < L; ++q)
                             ldx(c[q], xv[q]);
 #pragma unroll
                         for (int q = 0; q < L; ++q)
-                            if (j + q < j1)
+                            if (j + q < j1) {
+                                zstep(xv[q], dz[q]);
 #pragma unroll
                                 for (int u = 0; u < W; ++u)
                                     acc[u] = acc[u] + xv[q][u] * a[q];
+                            }
                     }
                 }
 #pragma unroll
@@ -178,7 +203,8 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         part[0][u] = part[0][u] + acc[u] * sv;
                         part[1][u] = part[1][u] + acc[u] * acc[u];
                         part[2][u] = part[2][u] + sv * sv;
-                    } else { // B_CHEB: t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
+                    } else if constexpr (OP == B_CHEB) {
+                        // t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
                         const double di = d.isd[g];
                         const double t2 = acc[u] * di;
                         double t1 = c1[u] * v.o1[o] + t2;
@@ -187,6 +213,18 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                             v.out2[o] = t1; // x is zero-flagged: x = t1
                         else
                             v.out[o] = t1;
+                    } else { // B_CHEBZ: r = b*isd ; t1 = c*r + (A u)*isd ; u' or x = t1*isd
+                        const double di = d.isd[g];
+                        const double rv = v.in[o] * di;
+                        const double t2 = acc[u] * di;
+                        double t1 = c1[u] * rv + t2;
+                        t1 = t1 * di;
+                        if (last) {
+                            v.out2[o] = t1;
+                        } else {
+                            v.out3[o] = rv;
+                            v.out[o] = t1;
+                        }
                     }
                 }
             } else {
@@ -412,11 +450,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchSolve P) {
             v.out = k == 1 ? x : P.cu1;
             v.out2 = x;
             v.out3 = P.cr;
-            batch_phase<B_CHEB0Z>(d, P.items, P.nitems, v, k - 1, k == 1, nullptr, wg, nwg);
+            int first = k - 2;
+            if (k >= 2) {
+                // the zero-x first step and the next one in one phase
+                BatchVecs z;
+                z.in = b;
+                z.out = P.cu1;
+                z.out2 = x;
+                z.out3 = P.cr;
+                batch_phase<B_CHEBZ>(d, P.items, P.nitems, z, k - 2, k == 2, nullptr, wg, nwg);
+                first = k - 3;
+            } else {
+                batch_phase<B_CHEB0Z>(d, P.items, P.nitems, v, k - 1, k == 1, nullptr, wg, nwg);
+            }
             if (!sync())
                 return false;
             double *uc = P.cu1, *un = P.cu2;
-            for (int i = k - 2; i >= 0; --i) {
+            for (int i = first; i >= 0; --i) {
                 BatchVecs w;
                 w.in = uc;
                 w.o1 = P.cr;
@@ -547,6 +597,7 @@ void batch_launch(BatchOp op, const BatchDev &d, const BatchItem *items, int nit
         XH_BATCH_CASE(B_DOT3)
         XH_BATCH_CASE(B_CHEB0Z)
         XH_BATCH_CASE(B_CHEB)
+        XH_BATCH_CASE(B_CHEBZ)
         XH_BATCH_CASE(B_COPY)
         XH_BATCH_CASE(B_S_UPDATE)
         XH_BATCH_CASE(B_XR_UPDATE)

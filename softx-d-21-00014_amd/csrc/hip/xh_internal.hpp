@@ -397,6 +397,9 @@ enum BatchOp : int {
     B_S_UPDATE = 6,  // out = b + nalpha o1
     B_XR_UPDATE = 7, // x = ac p_ + oc s_ + x ; out = s - omega t ; (out, o1)
     B_P_UPDATE = 8,  // out = b + pcoef o1 + beta out
+    B_CHEBZ = 9,     // B_CHEB0Z (coefficient ci + 1) and the B_CHEB step after it
+                     // (coefficient ci) in one launch: gathers in = b, each entry as
+                     // (c'*(b*isd))*isd; out3 = r = b*isd unless `last`
 };
 
 // Operands of the single-launch batched solve (k_batch_solve): the systems, their work

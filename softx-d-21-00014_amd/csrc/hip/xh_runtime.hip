@@ -2327,9 +2327,21 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
         v.out = k == 1 ? x : B->cu1;
         v.out2 = x;
         v.out3 = B->cr;
-        batch_launch(B_CHEB0Z, d, items, nit, B->grid, v, k - 1, k == 1, nullptr, st, na);
+        int first = k - 2;
+        if (k >= 2) {
+            // the zero-x first step and the next one in one launch (bit-identical)
+            BatchVecs z;
+            z.in = b;
+            z.out = B->cu1;
+            z.out2 = x;
+            z.out3 = B->cr;
+            batch_launch(B_CHEBZ, d, items, nit, B->grid, z, k - 2, k == 2, nullptr, st, na);
+            first = k - 3;
+        } else {
+            batch_launch(B_CHEB0Z, d, items, nit, B->grid, v, k - 1, k == 1, nullptr, st, na);
+        }
         double *uc = B->cu1, *un = B->cu2;
-        for (int i = k - 2; i >= 0; --i) {
+        for (int i = first; i >= 0; --i) {
             BatchVecs w;
             w.in = uc;
             w.o1 = B->cr;
