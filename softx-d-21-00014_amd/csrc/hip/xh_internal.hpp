@@ -258,6 +258,8 @@ enum SpmvOp : int {
                      // the operand is b, each gathered entry u[c] = (c2*(b[c]*isd[c]))*isd[c]
                      // (the zero-x first step), r = b*isd written unless last, then as
                      // OP_CHEB: t1 = c*r + (A u)*isd ; u' = t1*isd [or x = t1*isd when last]
+    OP_CHEBR = 12,   // OP_CHEB with r = b*isd formed from b (the steps after OP_CHEBZ,
+                     // which then need not store r)
 };
 
 struct SpmvArgs {

@@ -720,14 +720,16 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
         const int k = role.poly_order;
         void *uc = L.su1, *un = L.su2;
         int first = k - 2; // the first coefficient index left for the OP_CHEB loop
+        bool r_from_b = false; // the later steps form r = b*isd themselves (OP_CHEBR)
         if (x_zero && k >= 2 && g_tune.chebz_enable && L.A.idx != IDX_U8 &&
             spmv_row_parallel(L.A, OP_CHEB, nv)) {
             // the zero-x first step and the next one in one launch: the SpMV gathers b and
-            // forms u = (c*(b*isd))*isd per entry, bit-identical to the two launches
+            // forms u = (c*(b*isd))*isd per entry, bit-identical to the two launches; r is
+            // not stored, the later steps form it from b
             SpmvArgs a = args(S);
             a.b = b;
             a.isd = L.A.inv_sqrt_diag;
-            a.out2 = L.sr;
+            a.out2 = nullptr;
             a.c = cheb[k - 2];
             a.c2 = cheb[k - 1];
             a.last = k == 2;
@@ -735,6 +737,7 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
             a.x_zero = 1;
             spmv(L.A, nv, b, uc, OP_CHEBZ, a, st);
             first = k - 3;
+            r_from_b = true;
         } else if (x_zero) {
             cheb_init_zero(L.n, nv, L.prec, b, L.A.inv_sqrt_diag, cheb[k - 1], L.sr,
                            k == 1 ? x : uc, k == 1, S->dflag, st);
@@ -757,11 +760,12 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
             SpmvArgs a = args(S);
             a.isd = L.A.inv_sqrt_diag;
             a.r = L.sr;
+            a.b = b;
             a.c = cheb[i];
             a.last = i == 0;
             a.xupd = x;
             a.x_zero = x_zero ? 1 : 0;
-            spmv(L.A, nv, uc, un, OP_CHEB, a, st);
+            spmv(L.A, nv, uc, un, r_from_b ? OP_CHEBR : OP_CHEB, a, st);
             std::swap(uc, un);
         }
         return;

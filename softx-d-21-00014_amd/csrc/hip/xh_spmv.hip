@@ -294,9 +294,9 @@ __device__ __forceinline__ void epi_load(int64_t i, size_t o, size_t oy, int k0,
 #pragma unroll
         for (int w = 0; w < W; ++w)
             in.al[w] = rr[k0 + w];
-    } else if constexpr (OP == OP_CHEB0 || OP == OP_CHEB || OP == OP_CHEBZ) {
+    } else if constexpr (OP == OP_CHEB0 || OP == OP_CHEB || OP == OP_CHEBZ || OP == OP_CHEBR) {
         in.di = isd[i];
-        if constexpr (OP == OP_CHEB0 || OP == OP_CHEBZ)
+        if constexpr (OP == OP_CHEB0 || OP == OP_CHEBZ || OP == OP_CHEBR)
             in.a = ld<VT, W>(b + o);
         else
             in.a = ld<VT, W>(rr + o);
@@ -411,10 +411,15 @@ __device__ __forceinline__ void row_epilogue(
         if (!last)
             st<VT, W>(out2 + o, rv);
         st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
-    } else if constexpr (OP == OP_CHEB) {
+    } else if constexpr (OP == OP_CHEB || OP == OP_CHEBR) {
         // t2 = (A u)*isd ; t1 = c*r + t2 ; then u' = t1*isd or x (+)= t1*isd
+        // (OP_CHEBR: r = b*isd here)
         const VT di = in.di;
-        const P &r0 = in.a;
+        P r0 = in.a;
+        if constexpr (OP == OP_CHEBR)
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                r0.v[w] = r0.v[w] * di;
         P t1;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -429,8 +434,9 @@ __device__ __forceinline__ void row_epilogue(
         }
         st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
     } else if constexpr (OP == OP_CHEBZ) {
-        // r = b*isd (the zero-x first step's residual, kept for the later steps unless
-        // this step is the last) ; t2 = (A u)*isd ; t1 = c*r + t2 ; u' or x = t1*isd
+        // r = b*isd (the zero-x first step's residual, stored for the later steps when
+        // out2 is given and this step is not the last) ; t2 = (A u)*isd ; t1 = c*r + t2 ;
+        // u' or x = t1*isd
         const VT di = in.di;
         P rv, t1;
 #pragma unroll
@@ -440,7 +446,7 @@ __device__ __forceinline__ void row_epilogue(
             t1.v[w] = c * rv.v[w] + t2;
             t1.v[w] = t1.v[w] * di;
         }
-        if (!last)
+        if (!last && out2 != nullptr)
             st<VT, W>(out2 + o, rv);
         st<VT, W>((last ? (VT *)xupdv : (VT *)yv) + o, t1);
     } else if constexpr (OP == OP_PROLONG32) {
@@ -1155,7 +1161,7 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     constexpr int RPB = kBlock / (NV / W);
     const int cap = ND > 0 ? g_tune.dot_groups : (a.grid_cap > 0 ? a.grid_cap : 16384);
     if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
-                  OP != OP_CHEBZ && NV >= 8) {
+                  OP != OP_CHEBZ && OP != OP_CHEBR && NV >= 8) {
         if (A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr) {
             constexpr int G = 64 / (NV / W);
             const StLaunch S =
@@ -1172,7 +1178,7 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
     // column to skip the row's own block made the grid sweep slower (466 against 369 us on
     // the 96^3 27-point operator)
     if constexpr (sizeof(IT) != 1 && OP != OP_OFFBLOCK && OP != OP_PROLONG32 &&
-                  OP != OP_CHEBZ) {
+                  OP != OP_CHEBZ && OP != OP_CHEBR) {
         if (A.gcode != nullptr && grid_on(A) && A.row_ind == nullptr) {
             // the grid sweep's dot launches take their own partial count: more work items
             // pay there (27-point, 96^3: 409 -> 336 us with 2048), unlike the row path
@@ -1222,6 +1228,8 @@ int by_op(const Mat &A, const void *x, void *y, SpmvOp op, const SpmvArgs &a, hi
         return launch_t<VT, IT, NV, OP_CHEB>(A, x, y, a, s);
     case OP_CHEBZ:
         return launch_t<VT, IT, NV, OP_CHEBZ>(A, x, y, a, s);
+    case OP_CHEBR:
+        return launch_t<VT, IT, NV, OP_CHEBR>(A, x, y, a, s);
     case OP_OFFBLOCK:
         return launch_t<VT, IT, NV, OP_OFFBLOCK>(A, x, y, a, s);
     default:
@@ -1277,8 +1285,9 @@ namespace {
 const char *op_name(SpmvOp op) {
     static const char *names[] = {"OP_SET",  "OP_ADD",       "OP_RESID",    "OP_DOT1",
                                   "OP_DOT3", "OP_CHEB0",     "OP_CHEB",     "OP_PROLONG32",
-                                  "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4", "OP_CHEBZ"};
-    return (op >= 0 && op <= OP_CHEBZ) ? names[op] : "OP_?";
+                                  "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4", "OP_CHEBZ",
+                                  "OP_CHEBR"};
+    return (op >= 0 && op <= OP_CHEBR) ? names[op] : "OP_?";
 }
 
 int op_ndots(SpmvOp op) {
@@ -1288,12 +1297,12 @@ int op_ndots(SpmvOp op) {
 // the launch_t conditions for the 7-point stencil and the structured-grid kernels
 bool takes_st7(const Mat &A, SpmvOp op, int nv) {
     return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && op != OP_CHEBZ &&
-           nv >= 8 &&
+           op != OP_CHEBR && nv >= 8 &&
            A.st7 != nullptr && g_tune.st7_enable && A.row_ind == nullptr;
 }
 bool takes_grid(const Mat &A, SpmvOp op) {
     return A.idx != IDX_U8 && op != OP_OFFBLOCK && op != OP_PROLONG32 && op != OP_CHEBZ &&
-           A.gcode != nullptr && grid_on(A) && A.row_ind == nullptr;
+           op != OP_CHEBR && A.gcode != nullptr && grid_on(A) && A.row_ind == nullptr;
 }
 
 // Algorithmic HBM bytes of one SpMV launch: the matrix streams once (values, column
@@ -1343,8 +1352,13 @@ double spmv_bytes(const Mat &A, int nv, SpmvOp op, const SpmvArgs &a, bool grid,
         vec = V + n * sv + (a.last ? (a.x_zero ? V : 2.0 * V) : V);
         break;
     case OP_CHEBZ:
-        // b is the operand (counted with x) ; isd ; last: x written, else u' and r written
-        vec = n * sv + (a.last ? V : 2.0 * V);
+        // b is the operand (counted with x) ; isd ; last: x written, else u' (and r when
+        // stored) written
+        vec = n * sv + (a.last ? V : (a.out2 ? 2.0 * V : V));
+        break;
+    case OP_CHEBR:
+        // b, isd ; last: x written, else u' written
+        vec = V + n * sv + V;
         break;
     case OP_PROLONG32:
         vec = n * nv * 8.0 * (a.x_zero ? 1.0 : 2.0); // fp64 x written (and read)

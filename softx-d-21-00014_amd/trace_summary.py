@@ -16,7 +16,8 @@ import re
 from collections import defaultdict
 
 OPS = {"k_spmv": ["OP_SET", "OP_ADD", "OP_RESID", "OP_DOT1", "OP_DOT3", "OP_CHEB0", "OP_CHEB",
-                  "OP_PROLONG32", "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4", "OP_CHEBZ"]}
+                  "OP_PROLONG32", "OP_OFFBLOCK", "OP_RESID0", "OP_RDOT4", "OP_CHEBZ",
+                  "OP_CHEBR"]}
 VEC = ["V_S_UPDATE", "V_XR_UPDATE", "V_P_UPDATE", "V_GROUP1", "V_X_UPDATE", "V_R_UPDATE",
        "V_GROUP5", "V_DOT3", "V_DOT1", "V_CG_XR", "V_DOT2", "V_CG_P", "V_COPY_DOT", "V_COPY",
        "V_RB_UPDATE", "V_RB_UPDATE_MERGED"]
