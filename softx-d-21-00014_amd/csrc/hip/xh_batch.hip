@@ -204,10 +204,11 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         part[1][u] = part[1][u] + acc[u] * acc[u];
                         part[2][u] = part[2][u] + sv * sv;
                     } else if constexpr (OP == B_CHEB) {
-                        // t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
+                        // r = b*isd ; t1 = c*r + (A u)*isd ; u' = t1*isd or x = t1*isd
                         const double di = d.isd[g];
+                        const double rv = v.o1[o] * di;
                         const double t2 = acc[u] * di;
-                        double t1 = c1[u] * v.o1[o] + t2;
+                        double t1 = c1[u] * rv + t2;
                         t1 = t1 * di;
                         if (last)
                             v.out2[o] = t1; // x is zero-flagged: x = t1
@@ -219,12 +220,10 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         const double t2 = acc[u] * di;
                         double t1 = c1[u] * rv + t2;
                         t1 = t1 * di;
-                        if (last) {
+                        if (last)
                             v.out2[o] = t1;
-                        } else {
-                            v.out3[o] = rv;
+                        else
                             v.out[o] = t1;
-                        }
                     }
                 }
             } else {
@@ -237,12 +236,10 @@ __device__ __forceinline__ void batch_item(const BatchDev &d, const BatchItem &w
                         const double rv = v.b[o] * di;
                         double t1 = c1[u] * rv;
                         t1 = t1 * di;
-                        if (last) {
+                        if (last)
                             v.out2[o] = t1;
-                        } else {
-                            v.out3[o] = rv;
+                        else
                             v.out[o] = t1;
-                        }
                     } else if constexpr (OP == B_COPY) {
                         v.out[o] = v.b[o];
                     } else if constexpr (OP == B_S_UPDATE) {
@@ -449,7 +446,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchSolve P) {
             v.b = b;
             v.out = k == 1 ? x : P.cu1;
             v.out2 = x;
-            v.out3 = P.cr;
             int first = k - 2;
             if (k >= 2) {
                 // the zero-x first step and the next one in one phase
@@ -457,7 +453,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchSolve P) {
                 z.in = b;
                 z.out = P.cu1;
                 z.out2 = x;
-                z.out3 = P.cr;
                 batch_phase<B_CHEBZ>(d, P.items, P.nitems, z, k - 2, k == 2, nullptr, wg, nwg);
                 first = k - 3;
             } else {
@@ -469,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchSolve P) {
             for (int i = first; i >= 0; --i) {
                 BatchVecs w;
                 w.in = uc;
-                w.o1 = P.cr;
+                w.o1 = b;
                 w.out = un;
                 w.out2 = x;
                 batch_phase<B_CHEB>(d, P.items, P.nitems, w, i, i == 0, nullptr, wg, nwg);

@@ -381,10 +381,10 @@ struct BatchDev {
 struct BatchVecs {
     const double *in = nullptr;  // SpMV operand
     const double *b = nullptr;   // RESID0, CHEB0Z: b; COPY: source; S_/P_UPDATE: r
-    const double *o1 = nullptr;  // DOT1, XR_UPDATE: r0; DOT3: s; CHEB: scaled r; S_/P_UPDATE: v
+    const double *o1 = nullptr;  // DOT1, XR_UPDATE: r0; DOT3: s; CHEB: b; S_/P_UPDATE: v
     double *out = nullptr;       // result (P_UPDATE: p, read and written)
     double *out2 = nullptr;      // RESID0: p; CHEB0Z/CHEB with `last`: x
-    double *out3 = nullptr;      // RESID0: r; CHEB0Z: scaled r
+    double *out3 = nullptr;      // RESID0: r
     double *x = nullptr;         // XR_UPDATE
     const double *p_ = nullptr, *s_ = nullptr, *s = nullptr, *t = nullptr; // XR_UPDATE
 };
@@ -394,14 +394,15 @@ enum BatchOp : int {
     B_DOT1 = 1,      // out = A in ; (out, o1)
     B_DOT3 = 2,      // out = A in ; (out, o1), (out, out), (o1, o1)
     B_CHEB0Z = 3,    // first Chebyshev step from a zero x (coefficient ci)
-    B_CHEB = 4,      // one Horner step (coefficient ci), x = t1*isd when `last`
+    B_CHEB = 4,      // one Horner step (coefficient ci) with r = o1*isd formed from o1 = b,
+                     // x = t1*isd when `last`
     B_COPY = 5,      // out = b
     B_S_UPDATE = 6,  // out = b + nalpha o1
     B_XR_UPDATE = 7, // x = ac p_ + oc s_ + x ; out = s - omega t ; (out, o1)
     B_P_UPDATE = 8,  // out = b + pcoef o1 + beta out
     B_CHEBZ = 9,     // B_CHEB0Z (coefficient ci + 1) and the B_CHEB step after it
                      // (coefficient ci) in one launch: gathers in = b, each entry as
-                     // (c'*(b*isd))*isd; out3 = r = b*isd unless `last`
+                     // (c'*(b*isd))*isd
 };
 
 // Operands of the single-launch batched solve (k_batch_solve): the systems, their work
@@ -422,8 +423,7 @@ struct BatchSolve {
     const int *item_start = nullptr, *item_count = nullptr, *nvs = nullptr;
     double *partials = nullptr;
     double *x = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr, *s = nullptr;
-    double *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cr = nullptr, *cu1 = nullptr,
-           *cu2 = nullptr;
+    double *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cu1 = nullptr, *cu2 = nullptr;
     const double *b = nullptr;
 };
 // workgroups of k_batch_solve that are resident at once (0 when the query fails)

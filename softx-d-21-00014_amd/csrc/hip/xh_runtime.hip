@@ -2223,8 +2223,8 @@ struct xh_batch_s {
     SolverState *st = nullptr;
     double *partials = nullptr;
     double *x = nullptr, *b = nullptr, *r0 = nullptr, *r = nullptr, *p = nullptr, *v = nullptr;
-    double *s = nullptr, *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cr = nullptr,
-           *cu1 = nullptr, *cu2 = nullptr;
+    double *s = nullptr, *t = nullptr, *p_ = nullptr, *s_ = nullptr, *cu1 = nullptr,
+           *cu2 = nullptr;
     unsigned long long *d_t0 = nullptr;
     BatchBarrier *d_bar = nullptr; // grid barriers of the single-launch solve
     // host
@@ -2330,7 +2330,6 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
         v.b = b;
         v.out = k == 1 ? x : B->cu1;
         v.out2 = x;
-        v.out3 = B->cr;
         int first = k - 2;
         if (k >= 2) {
             // the zero-x first step and the next one in one launch (bit-identical)
@@ -2338,7 +2337,6 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
             z.in = b;
             z.out = B->cu1;
             z.out2 = x;
-            z.out3 = B->cr;
             batch_launch(B_CHEBZ, d, items, nit, B->grid, z, k - 2, k == 2, nullptr, st, na);
             first = k - 3;
         } else {
@@ -2348,7 +2346,7 @@ void batch_precond(xh_batch_s *B, const double *b, double *x) {
         for (int i = first; i >= 0; --i) {
             BatchVecs w;
             w.in = uc;
-            w.o1 = B->cr;
+            w.o1 = b;
             w.out = un;
             w.out2 = x;
             batch_launch(B_CHEB, d, items, nit, B->grid, w, i, i == 0, nullptr, st, na);
@@ -2431,7 +2429,7 @@ void free_batch(xh_batch_s *B) {
                     (void *)B->d_items, (void *)B->d_items_act, (void *)B->d_nact,
                     (void *)B->st, (void *)B->partials, (void *)B->x,
                     (void *)B->b, (void *)B->r0, (void *)B->r, (void *)B->p, (void *)B->v,
-                    (void *)B->s, (void *)B->t, (void *)B->p_, (void *)B->s_, (void *)B->cr,
+                    (void *)B->s, (void *)B->t, (void *)B->p_, (void *)B->s_,
                     (void *)B->cu1, (void *)B->cu2, (void *)B->d_t0, (void *)B->d_bar})
         if (q)
             (void)hipFree(q);
@@ -2550,7 +2548,7 @@ extern "C" int xh_batch_create(int nsys, const int64_t *nrows, const int32_t *nv
         (rc = dmalloc(&B->d_bar, 1, acc)))
         return bail(rc);
     for (double **q : {&B->x, &B->b, &B->r0, &B->r, &B->p, &B->v, &B->s, &B->t, &B->p_, &B->s_,
-                       &B->cr, &B->cu1, &B->cu2})
+                       &B->cu1, &B->cu2})
         if ((rc = dmalloc(q, ve, acc)))
             return bail(rc);
     hipError_t e = hipSuccess;
@@ -2648,7 +2646,6 @@ extern "C" int xh_batch_solve(xh_batch B) {
         P.t = B->t;
         P.p_ = B->p_;
         P.s_ = B->s_;
-        P.cr = B->cr;
         P.cu1 = B->cu1;
         P.cu2 = B->cu2;
         const int cap = batch_solve_blocks();
@@ -2812,7 +2809,7 @@ extern "C" int xh_batch_autotune(xh_batch B, int reps, int32_t *tile_passes) {
     return upload_items(B);
 }
 
-// Times the batched Chebyshev step (B_CHEB: y = isd (c r + isd (A u)) over every work item
+// Times the batched Chebyshev step (B_CHEB: y = isd (c (b isd) + isd (A u)) over every work item
 // of every system, as in the preconditioner) with every system active: reps launches
 // between two events on the batch stream.  The systems' solver state (device and its host
 // copy) is saved first and restored afterwards; the work vector t is overwritten.
