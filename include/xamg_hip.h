@@ -156,7 +156,10 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
  * the rows average 12 or more entries, e.g. 27-point operators), "spmv_grid_items" (default 2048),
  * "cheb_fuse_zero" (0/1, default 1: a Chebyshev smoother from a zero x runs its first two
  * steps as one SpMV that gathers b and forms the first step's values per entry,
- * bit-identical to two launches),
+ * bit-identical to two launches), "hsgs_offblock_csr" (0/1, default 1, read at solver
+ * setup: a hybrid GS level also keeps the entries outside each row's core block as a
+ * matrix of their own and computes the off-block residual as b - (that matrix) x,
+ * bit-identical to skipping the in-block columns of A),
  * "spmv_grid_min_seg" (default 32), "unet_fuse_up" (0/1, default 1: U-Net inference reads
  * the decoder's upsampled inputs through the convolution instead of writing them),
  * "unet_conv_tile" (0/1, default 1: forward convolutions with 16 or 32 output channels

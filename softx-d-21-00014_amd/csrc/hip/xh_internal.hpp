@@ -175,6 +175,10 @@ struct Tuning {
     // 1: a Chebyshev smoother from a zero x runs its first two steps as one SpMV
     // (OP_CHEBZ) where OP_CHEB would run row-parallel
     int chebz_enable = 1;
+    // 1: hybrid GS levels keep the entries outside each row's core block as a matrix of
+    // their own (built at setup) and run the off-block residual as an OP_RESID over it
+    // (grid path for 27-point operators); 0: OP_OFFBLOCK over A, skipping in-block columns
+    int hsgs_off_csr = 1;
     int grid_items = 2048;          // target work items (tile x segment) per launch
     int grid_min_seg = 32;          // shortest x segment of a work item
     // 1: matrices with a complete 7-point stencil view run the register-window line

@@ -442,6 +442,26 @@ int upload_mat(const HostCsr &h, int prec, int index_mode, Mat &M) {
 // value +0: the sweep kernel reads a zero x there, so a pad subtracts +0 * +0 = +0 from
 // the row sum, which leaves every value (-0 included) unchanged.  KB (3 or 4) is the width
 // that needs fewer pads.
+// The entries of each row i outside its core block [lo, lo + B), lo = (i / B) * B, in
+// stored order: b - (this matrix) x sums the products of the off-block residual of hybrid
+// GS (csr_gs.inl:75-130, the columns outside the calling core's rows) in their order.
+HostCsr off_block_csr(const HostCsr &h, int64_t B) {
+    HostCsr o;
+    o.nrows = h.nrows;
+    o.ncols = h.ncols;
+    o.row.assign((size_t)h.nrows + 1, 0);
+    for (int64_t i = 0; i < h.nrows; ++i) {
+        const int64_t lo = (i / B) * B;
+        for (uint32_t j = h.row[i]; j < h.row[i + 1]; ++j)
+            if ((int64_t)h.col[j] < lo || (int64_t)h.col[j] >= lo + B) {
+                o.col.push_back(h.col[j]);
+                o.val.push_back(h.val[j]);
+            }
+        o.row[i + 1] = (uint32_t)o.col.size();
+    }
+    return o;
+}
+
 int upload_core(const HostCsr &h, int64_t B, int prec, const Mat &A, Mat &M) {
     M = Mat();
     const int64_t n = h.nrows;
@@ -653,6 +673,9 @@ struct Level {
     // columns; inv_diag is A's): what the hybrid GS sweeps read, built when a role of
     // the level is HSGS
     Mat core;
+    // the other entries of A (outside each row's core block, stored order): b - off x is
+    // the off-block residual of the sweeps; built with `core` when hsgs_offblock_csr is on
+    Mat off;
     void *tmp = nullptr, *tmp_t = nullptr, *b = nullptr, *x = nullptr;
     void *sr = nullptr, *su1 = nullptr, *su2 = nullptr;
     void *dinv = nullptr;
@@ -795,8 +818,12 @@ void smooth(xh_solver_s *S, const xh_role &role, Level &L, const std::vector<dou
             } else {
                 SpmvArgs a = args(S);
                 a.b = b;
-                a.blk_rows = L.hsgs_B;
-                spmv(L.A, nv, x, L.sr, OP_OFFBLOCK, a, st);
+                if (L.off.row && g_tune.hsgs_off_csr) {
+                    spmv(L.off, nv, x, L.sr, OP_RESID, a, st);
+                } else {
+                    a.blk_rows = L.hsgs_B;
+                    spmv(L.A, nv, x, L.sr, OP_OFFBLOCK, a, st);
+                }
                 hsgs_sweeps(G, nv, L.sr, x, 0, relax, L.hsgs_B, S->dflag, st);
             }
             x_zero = false;
@@ -1606,6 +1633,11 @@ int xh_solver_create(xh_hier H, const xh_params *prm, int nv, xh_solver *out) {
             if ((rc = upload_core(H->A[l], L.hsgs_B, L.prec, L.A, L.core)))
                 return bail(rc);
             S->bytes += L.core.bytes;
+            if (g_tune.hsgs_off_csr) {
+                if ((rc = upload_mat(off_block_csr(H->A[l], L.hsgs_B), L.prec, 0, L.off)))
+                    return bail(rc);
+                S->bytes += L.off.bytes;
+            }
         }
     }
     // coarsest direct solve: dense Gauss-Jordan inverse on the host (Direct::init)
@@ -1843,6 +1875,7 @@ int xh_solver_destroy(xh_solver S) {
             free_mat(L.A);
         L.core.inv_diag = nullptr; // A's
         free_mat(L.core);
+        free_mat(L.off);
         free_mat(L.P);
         free_mat(L.R);
         for (void *q : {L.tmp, L.tmp_t, L.b, L.x, L.sr, L.su1, L.su2, L.dinv, L.cb64, L.cx64})
@@ -1936,6 +1969,8 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.fold_min_bytes = value;
     else if (k == "cheb_fuse_zero" && (value == 0 || value == 1))
         g_tune.chebz_enable = (int)value;
+    else if (k == "hsgs_offblock_csr" && (value == 0 || value == 1))
+        g_tune.hsgs_off_csr = (int)value;
     else if (k == "spmv_grid_enable" && value >= 0 && value <= 2)
         g_tune.grid_enable = (int)value;
     else if (k == "spmv_st7_enable" && (value == 0 || value == 1))

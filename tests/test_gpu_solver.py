@@ -189,6 +189,38 @@ def test_fused_zero_x_chebyshev_step_is_identical(gpu, k, mg_prec):
 
 
 @pytest.mark.parametrize("mg_prec", [0, 2])
+def test_hsgs_offblock_matrix_is_identical(gpu, mg_prec):
+    # the off-block residual of hybrid GS over the off-block entries kept as a matrix of
+    # their own (OP_RESID, the grid path for the 27-point operator) against OP_OFFBLOCK
+    # over A, which skips the in-block columns: the same products in the same order
+    X = gpu
+    ops = (("cube", X.cube(12)[:3]), ("q1fem27", X.q1fem27(12, 12, 12)[:3]))
+    try:
+        for name, (row, col, val) in ops:
+            levels = X.Amg(row, col, val).export()
+            n = len(row) - 1
+            prm = X.default_params(method="MergedPBiCGStab", smoother="HSGS", sweeps=2,
+                                   mg_prec=mg_prec)
+            for nv in (1, 4, 8, 32):
+                print(f"case {name} mg_prec {mg_prec} nv {nv}", flush=True)
+                b = X.rhs_uniform(nv + 3, n, nv)
+                out, names = [], []
+                for on in (0, 1):
+                    X.set_option("hsgs_offblock_csr", on)
+                    S = X.Solver(X.Hier(levels), prm, nv)
+                    db, dx = X.DeviceArray.from_host(b), X.DeviceArray(n * nv)
+                    S.precond_apply(db, dx)
+                    out.append(dx.to_host())
+                    dx.zero()
+                    names.append({q["kernel"] for q in S.profile(db, dx, 1)})
+                assert np.array_equal(out[0], out[1]), (name, nv)
+                assert any("OP_OFFBLOCK" in q for q in names[0]), names[0]
+                assert not any("OP_OFFBLOCK" in q for q in names[1]), names[1]
+    finally:
+        X.set_option("hsgs_offblock_csr", 1)
+
+
+@pytest.mark.parametrize("mg_prec", [0, 2])
 def test_hsgs_core_path_bit_exact(gpu, O, mg_prec):
     # blocks small enough for the staged core-block sweep in both precisions (32-row
     # blocks: split_range gives 40 rows); the launches of a solve show that kernel ran
