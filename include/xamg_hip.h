@@ -147,7 +147,7 @@ int xh_csr_fold(int64_t nrows, int64_t ncols, const uint32_t *row, const uint32_
 int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv, int dots,
                      int64_t *grid, int64_t *ntiles, int64_t *tpw, int64_t *rows);
 /* Launch-shape knobs: "spmv_strip_budget" (bytes, default 8 MiB), "spmv_tile_passes"
- * (default 8), "spmv_min_passes" (default 1: in row order, at least this many passes of
+ * (default 2), "spmv_min_passes" (default 1: in row order, at least this many passes of
  * rows per workgroup), "dot_groups" (default 1024, a multiple of 8 up to 4096: workgroups
  * of a launch that writes dot-product partials), "grid_dot_groups" (default 2048: the same
  * for the structured-grid SpMV), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only

@@ -161,7 +161,11 @@ constexpr int64_t kGridMinRowNnz = 12;
 
 struct Tuning {
     int64_t strip_budget = 8 << 20; // bytes of x+y+A touched between reuses of an x row
-    int tile_passes = 8;            // tile rows = tile_passes * rows per workgroup pass
+    // tile rows = tile_passes * rows per workgroup pass.  Short tiles keep the rows an
+    // XCD's resident workgroups hold in flight (~224 of them) within what its L2 serves:
+    // the 64^3 coarse level of the 128^3 cube (26 entries per row) takes 115 us per
+    // Chebyshev step with 8 passes, 99 with 2; the finest level is flat from 2 to 8
+    int tile_passes = 2;
     int min_passes = 1;             // row-order schedule: passes per workgroup at least
     int dot_groups = 1024;          // workgroups of a launch that writes dot partials
     int fold_enable = 1;            // 0: always contiguous order
