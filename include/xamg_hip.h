@@ -150,7 +150,8 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
  * (default 2), "spmv_min_passes" (default 1: in row order, at least this many passes of
  * rows per workgroup), "dot_groups" (default 1024, a multiple of 8 up to 4096: workgroups
  * of a launch that writes dot-product partials), "grid_dot_groups" (default 2048: the same
- * for the structured-grid SpMV), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
+ * for the structured-grid SpMV), "vec_dot_groups" (default 2048: the same for the fused
+ * vector updates), "spmv_fold_enable" (0/1), "spmv_fold_min_bytes" (default 2 MiB: fold only
  * when 2*fold*nv*sizeof(value) exceeds it), "spmv_grid_enable" (the
  * structured-grid SpMV of matrices with a grid view: 0 never, 1 always, 2 = default: when
  * the rows average 12 or more entries, e.g. 27-point operators), "spmv_grid_items" (default 2048),

@@ -168,6 +168,10 @@ struct Tuning {
     int tile_passes = 2;
     int min_passes = 1;             // row-order schedule: passes per workgroup at least
     int dot_groups = 1024;          // workgroups of a launch that writes dot partials
+    // the same for the fused vector updates with dot partials (k_vec): a pure stream,
+    // which more workgroups serve better (V_XR_UPDATE at 128^3, nv = 32: 957 us with 1024,
+    // 833 with 2048), unlike the SpMMs (465 against 503-521 us)
+    int vec_dot_groups = 2048;
     int fold_enable = 1;            // 0: always contiguous order
     int64_t fold_min_bytes = 2 << 20; // fold only when 2*fold*nv*sizeof(value) exceeds this
     // structured-grid matrices run the LDS-window line sweep (k_spmv_grid): 0 never, 1

@@ -1963,6 +1963,8 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.dot_groups = (int)value;
     else if (k == "grid_dot_groups" && value >= 8 && value <= kMaxPartials && value % 8 == 0)
         g_tune.grid_dot_groups = (int)value;
+    else if (k == "vec_dot_groups" && value >= 8 && value <= kMaxPartials && value % 8 == 0)
+        g_tune.vec_dot_groups = (int)value;
     else if (k == "spmv_fold_enable" && (value == 0 || value == 1))
         g_tune.fold_enable = (int)value;
     else if (k == "spmv_fold_min_bytes" && value >= 0)

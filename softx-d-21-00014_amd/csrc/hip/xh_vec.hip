@@ -194,7 +194,7 @@ int vec_nv(VecOp op, int64_t n, const VecArgs &a, int gate, hipStream_t s) {
     const bool dots = (op == V_XR_UPDATE || op == V_R_UPDATE || op == V_DOT1 || op == V_DOT3 ||
                        op == V_DOT2 || op == V_COPY_DOT);
     int64_t need = (n + rpb - 1) / rpb;
-    int64_t g = std::min<int64_t>(need, dots ? g_tune.dot_groups : 8192);
+    int64_t g = std::min<int64_t>(need, dots ? g_tune.vec_dot_groups : 8192);
     g = std::max<int64_t>(8, (g + 7) / 8 * 8);
     int64_t chunk = (n + g - 1) / g;
     chunk = (chunk + rpb - 1) / rpb * rpb;
