@@ -67,7 +67,9 @@ struct Mat {
 // A matrix whose rows couple mostly to rows +-fold away (a structured-grid operator,
 // fold = nx*ny) then reuses each x row within one strip a few tiles after its first
 // use, inside one XCD's L2, instead of fold rows later.  fold = strip = nrows, na = 1
-// is plain contiguous order.  Workgroup w processes tiles [w*tpw, (w+1)*tpw).
+// is plain contiguous order.  The m = grid/8 workgroups of XCD x share the tiles
+// [x*m*tpw, (x+1)*m*tpw): workgroup lb takes [lb*tpw, (lb+1)*tpw), or, in launches that
+// write dot partials, every m-th tile of the range (tile_of).
 struct Sched {
     int64_t nrows = 0, fold = 0, strip = 0, na = 1, ntiles = 0;
     int32_t tile = 0, tps = 0, tpw = 1;
@@ -92,6 +94,16 @@ __host__ __device__ inline void tile_rows(const Sched &s, int64_t k, int64_t &b,
         e = s.nrows;
     if (e < b)
         e = b;
+}
+
+// tile kt (0 <= kt < tpw) of logical workgroup lb in a grid of G workgroups (a multiple
+// of 8): workgroup q = lb % m of XCD x = lb / m (m = G/8) takes tiles q, q + m, q + 2m, ...
+// of its XCD's range, so the tiles in flight together on an XCD are consecutive and their
+// x rows are shared through its L2 (with each workgroup on tpw consecutive tiles instead,
+// the resident workgroups of an XCD sat a plane apart)
+__host__ __device__ inline int64_t tile_of(const Sched &s, int64_t lb, int kt, int64_t G) {
+    const int64_t m = G / 8, x = lb / m, q = lb - x * m;
+    return (x * s.tpw + kt) * m + q;
 }
 
 struct Launch {

@@ -486,9 +486,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
             part[d][w] = 0.0;
 
     for (int kt = 0; kt < sc.tpw; ++kt) {
-    const int64_t k = lb * sc.tpw + kt;
+    // launches with dot partials run few workgroups of many tiles each, which walk the
+    // tiles of their XCD together; the others take consecutive tiles
+    const int64_t k = ND > 0 ? tile_of(sc, lb, kt, gridDim.x) : lb * sc.tpw + kt;
     if (k >= sc.ntiles)
-        break;
+        break; // later kt give larger k
     int64_t start, end;
     tile_rows(sc, k, start, end);
     // Pipeline: the row pointers and panel base of the row two passes ahead and the
