@@ -160,7 +160,10 @@ int xh_spmv_schedule(int64_t nrows, int64_t nnz, int64_t fold, int prec, int nv,
  * bit-identical to two launches), "hsgs_offblock_csr" (0/1, default 1, read at solver
  * setup: a hybrid GS level also keeps the entries outside each row's core block as a
  * matrix of their own and computes the off-block residual as b - (that matrix) x,
- * bit-identical to skipping the in-block columns of A),
+ * bit-identical to skipping the in-block columns of A), "spmv_wide_batches" (0/1,
+ * default 1: SpMVs at nv 8 and 32 over matrices of few rows and 32 or more entries per row
+ * on average, at most 512 workgroups, issue 16 x loads per batch instead of 8; the sums
+ * and their order are unchanged),
  * "spmv_grid_min_seg" (default 32), "unet_fuse_up" (0/1, default 1: U-Net inference reads
  * the decoder's upsampled inputs through the convolution instead of writing them),
  * "unet_conv_tile" (0/1, default 1: forward convolutions with 16 or 32 output channels

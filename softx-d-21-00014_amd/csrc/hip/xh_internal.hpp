@@ -199,6 +199,8 @@ struct Tuning {
     // their own (built at setup) and run the off-block residual as an OP_RESID over it
     // (grid path for 27-point operators); 0: OP_OFFBLOCK over A, skipping in-block columns
     int hsgs_off_csr = 1;
+    // 1: few-row, long-row SpMVs (nv 8 and 32) issue 16 x loads per batch (k_spmv<..., 16>)
+    int wide_batches = 1;
     int grid_items = 2048;          // target work items (tile x segment) per launch
     int grid_min_seg = 32;          // shortest x segment of a work item
     // 1: matrices with a complete 7-point stencil view run the register-window line

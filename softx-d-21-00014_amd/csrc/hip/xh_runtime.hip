@@ -1973,6 +1973,8 @@ extern "C" int xh_set_option(const char *key, int64_t value) {
         g_tune.chebz_enable = (int)value;
     else if (k == "hsgs_offblock_csr" && (value == 0 || value == 1))
         g_tune.hsgs_off_csr = (int)value;
+    else if (k == "spmv_wide_batches" && (value == 0 || value == 1))
+        g_tune.wide_batches = (int)value;
     else if (k == "spmv_grid_enable" && value >= 0 && value <= 2)
         g_tune.grid_enable = (int)value;
     else if (k == "spmv_st7_enable" && (value == 0 || value == 1))

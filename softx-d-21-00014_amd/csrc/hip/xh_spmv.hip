@@ -71,9 +71,11 @@ __device__ __forceinline__ uint32_t panel_base(const uint32_t *__restrict__ pb, 
 // panel base), so a row of up to CH nonzeros costs one exposed memory round trip: its x
 // loads.  Only existing entries are added (no +0 terms), so each RHS gets the
 // reference's sequential sum exactly.
-template <typename VT, int NV, int W>
+// BX: x loads in flight per batch (8; 16 for the few-row, long-row launches, whose
+// workgroups leave the GPU mostly idle and whose time is the chain of batches of a row).
+template <typename VT, int NV, int W, int BX = 8>
 struct Chunk {
-    static constexpr int BW = 8;
+    static constexpr int BW = BX;
     static constexpr int L = NV / W;                    // lanes sharing the row
     static constexpr int LL = L < 16 ? L : 16;          // of which load the chunk
     static constexpr int E = L >= 2 ? (BW + LL - 1) / LL : BW; // entries per loading lane
@@ -85,14 +87,14 @@ struct Chunk {
 
 // SCALE: also the column's scale sd[base + c] of each loaded entry (a load that depends on
 // the column's), broadcast with the column and value
-template <typename VT, typename IT, int NV, int W, bool SCALE = false>
+template <typename VT, typename IT, int NV, int W, bool SCALE = false, int BX = 8>
 __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__restrict__ col,
-                                           const VT *__restrict__ val, Chunk<VT, NV, W> &ch,
-                                           uint32_t base = 0,
+                                           const VT *__restrict__ val,
+                                           Chunk<VT, NV, W, BX> &ch, uint32_t base = 0,
                                            const VT *__restrict__ sd = nullptr) {
     // Entries past the row end repeat the row's last entry (its x row exists), so the
     // x loads of a batch need no select; their products are never added.
-    using C = Chunk<VT, NV, W>;
+    using C = Chunk<VT, NV, W, BX>;
     const int gl = C::L >= 2 ? (int)(threadIdx.x & 63) % C::L : 0;
     const bool live = j < e && gl < C::LL;
 #pragma unroll
@@ -118,12 +120,13 @@ __device__ __forceinline__ void load_chunk(uint32_t j, uint32_t e, const IT *__r
 // loaded nor added), which keeps the order of the others.  SCALE: each gathered value
 // x[c] enters as (cz*(x[c]*d))*d with d the column's scale from the chunk (the zero-x
 // first Chebyshev step of OP_CHEBZ, rounded as that step rounds it).
-template <typename VT, int NV, int W, bool SKIP = false, bool SCALE = false>
-__device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chunk<VT, NV, W> &ch,
+template <typename VT, int NV, int W, bool SKIP = false, bool SCALE = false, int BX = 8>
+__device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e,
+                                              const Chunk<VT, NV, W, BX> &ch,
                                               uint32_t base, const VT *__restrict__ x, int k0,
                                               VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0,
                                               VT cz = VT(0)) {
-    using C = Chunk<VT, NV, W>;
+    using C = Chunk<VT, NV, W, BX>;
     constexpr int BW = C::BW, L = C::L;
     auto kept = [&](uint32_t cabs) { return !SKIP || cabs - slo >= sn; };
     auto scale = [&](Pack<VT, W> &xv, VT dq) __attribute__((always_inline)) {
@@ -188,14 +191,15 @@ __device__ __forceinline__ void consume_chunk(uint32_t j, uint32_t e, const Chun
 
 // the whole row [j, e), its first chunk already in `first` (SKIP: without the columns
 // in [slo, slo + sn))
-template <typename VT, typename IT, int NV, int W, bool SKIP = false, bool SCALE = false>
+template <typename VT, typename IT, int NV, int W, bool SKIP = false, bool SCALE = false,
+          int BX = 8>
 __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
-                                        const Chunk<VT, NV, W> &first,
+                                        const Chunk<VT, NV, W, BX> &first,
                                         const IT *__restrict__ col, const VT *__restrict__ val,
                                         uint32_t base, const VT *__restrict__ x, int k0,
                                         VT (&acc)[W], uint32_t slo = 0, uint32_t sn = 0,
                                         const VT *__restrict__ sd = nullptr, VT cz = VT(0)) {
-    using C = Chunk<VT, NV, W>;
+    using C = Chunk<VT, NV, W, BX>;
     // a longer row loads its next chunk before the x loads of the current one, so the
     // chunk's latency overlaps them
     C cur = first;
@@ -203,8 +207,8 @@ __device__ __forceinline__ void row_acc(uint32_t j, const uint32_t e,
         const bool more = j + C::CH < e; // the same for all lanes of the row
         C nxt;
         if (more)
-            load_chunk<VT, IT, NV, W, SCALE>(j + C::CH, e, col, val, nxt, base, sd);
-        consume_chunk<VT, NV, W, SKIP, SCALE>(j, e, cur, base, x, k0, acc, slo, sn, cz);
+            load_chunk<VT, IT, NV, W, SCALE, BX>(j + C::CH, e, col, val, nxt, base, sd);
+        consume_chunk<VT, NV, W, SKIP, SCALE, BX>(j, e, cur, base, x, k0, acc, slo, sn, cz);
         if (!more)
             break;
         j += C::CH;
@@ -460,7 +464,7 @@ __device__ __forceinline__ void row_epilogue(
     }
 }
 
-template <typename VT, typename IT, int NV, int OP>
+template <typename VT, typename IT, int NV, int OP, int BX = 8>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const uint32_t *__restrict__ row, const IT *__restrict__ col, const VT *__restrict__ val,
     const uint32_t *__restrict__ pbase, int shift, const Sched sc,
@@ -508,16 +512,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         nbs = panel_base<IT>(pbase, shift, i + RPB);
     }
     constexpr bool SC = OP == OP_CHEBZ; // chunks carry their columns' isd
-    Chunk<VT, NV, W> ch;
-    load_chunk<VT, IT, NV, W, SC>(jb, je, col, val, ch, bs, isd);
+    Chunk<VT, NV, W, BX> ch;
+    load_chunk<VT, IT, NV, W, SC, BX>(jb, je, col, val, ch, bs, isd);
     // Drain the prologue loads once per tile: the row loop's head then merges only its
     // own back edge, where the last y store may stay in flight.
     __builtin_amdgcn_s_waitcnt(0);
     for (; i < end; i += RPB) {
         const size_t o = (size_t)i * NV + k0;
         const uint32_t base = bs;
-        Chunk<VT, NV, W> nch;
-        load_chunk<VT, IT, NV, W, SC>(nb, ne, col, val, nch, nbs, isd);
+        Chunk<VT, NV, W, BX> nch;
+        load_chunk<VT, IT, NV, W, SC, BX>(nb, ne, col, val, nch, nbs, isd);
         uint32_t nnb = 0, nne = 0, nnbs = 0;
         if (i + 2 * RPB < end) {
             nnb = row[i + 2 * RPB];
@@ -542,14 +546,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         if constexpr (OP == OP_OFFBLOCK) {
             // t = b - sum over off-block columns (old x), HSGS core blocks of blk_rows
             const uint32_t lo = (uint32_t)((i / blk_rows) * blk_rows);
-            row_acc<VT, IT, NV, W, true>(jb, je, ch, col, val, base, x, k0, acc, lo,
+            row_acc<VT, IT, NV, W, true, false, BX>(jb, je, ch, col, val, base, x, k0, acc, lo,
                                          (uint32_t)blk_rows);
         } else if constexpr (OP == OP_CHEBZ) {
             // the operand is b; each gathered entry enters as the zero-x first step's u
-            row_acc<VT, IT, NV, W, false, true>(jb, je, ch, col, val, base, x, k0, acc, 0u, 0u,
+            row_acc<VT, IT, NV, W, false, true, BX>(jb, je, ch, col, val, base, x, k0, acc, 0u, 0u,
                                                 isd, c2);
         } else {
-            row_acc<VT, IT, NV, W>(jb, je, ch, col, val, base, x, k0, acc);
+            row_acc<VT, IT, NV, W, false, false, BX>(jb, je, ch, col, val, base, x, k0, acc);
         }
         EpiIn<VT, W> in;
         epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last, x_zero);
@@ -1154,6 +1158,16 @@ StLaunch st7_plan(int nx, int ny, int nz, int gy, int gz, int cap) {
     return S;
 }
 
+// Few rows of many entries (the coarse levels of an aggressively coarsened hierarchy: a
+// launch of at most 512 workgroups, rows averaging 32 entries or more) run 16 x loads per
+// batch instead of 8: such a launch leaves most of the GPU idle, and its time is the chain
+// of a row's batches (about a microsecond each), not bandwidth.  With ~1000 workgroups
+// the wider kernel's registers (104 against 78 VGPRs) already cost more than it saves.
+bool spmv_wide_batches(const Mat &A, int rpb) {
+    return g_tune.wide_batches && A.row_ind == nullptr && A.nnz >= 32 * A.nrows &&
+           (A.nrows + rpb - 1) / rpb <= 512;
+}
+
 namespace {
 
 template <typename VT, typename IT, int NV, int OP>
@@ -1196,6 +1210,20 @@ int launch_t(const Mat &A, const void *x, void *y, const SpmvArgs &a, hipStream_
         }
     }
     const Launch L = spmv_plan(A.nrows, A.nnz, A.fold, RPB, cap, (int64_t)NV * (int64_t)sizeof(VT));
+    if constexpr (sizeof(IT) != 1 && (NV == 8 || NV == 32) &&
+                  (OP == OP_SET || OP == OP_ADD || OP == OP_RESID || OP == OP_CHEB0 ||
+                   OP == OP_CHEB || OP == OP_CHEBZ || OP == OP_CHEBR)) {
+        if (spmv_wide_batches(A, RPB)) {
+            hipLaunchKernelGGL((k_spmv<VT, IT, NV, OP, 16>), dim3(L.grid), dim3(kBlock), 0, s,
+                               A.row, (const IT *)A.col, (const VT *)A.val, A.panel_base,
+                               A.panel_shift, L.sc, (const VT *)x, y, (const VT *)a.b,
+                               (const VT *)a.o1, (const VT *)a.isd, (const VT *)a.r,
+                               (VT *)a.out2, (VT *)a.out3, a.xupd, (VT)a.c, a.last, a.x_zero,
+                               a.out_f32, a.partials, a.done, a.blk_rows, A.row_ind,
+                               (VT)a.c2);
+            return ND > 0 ? L.grid : 0;
+        }
+    }
     hipLaunchKernelGGL((k_spmv<VT, IT, NV, OP>), dim3(L.grid), dim3(kBlock), 0, s, A.row,
                        (const IT *)A.col, (const VT *)A.val, A.panel_base, A.panel_shift,
                        L.sc, (const VT *)x, y, (const VT *)a.b, (const VT *)a.o1,
@@ -1406,7 +1434,11 @@ int spmv(const Mat &A, int nv, const void *x, void *y, SpmvOp op, const SpmvArgs
     if (led.on) {
         const bool st7 = takes_st7(A, op, nv);
         const bool grid = !st7 && takes_grid(A, op);
-        const std::string tail = std::to_string(nv) + "," + op_name(op) + ">";
+        const bool wide = !st7 && !grid && (nv == 8 || nv == 32) && A.idx != IDX_U8 &&
+                          (op == OP_SET || op == OP_ADD || op == OP_RESID || op == OP_CHEB0 ||
+                           op == OP_CHEB || op == OP_CHEBZ || op == OP_CHEBR) &&
+                          spmv_wide_batches(A, kBlock / (nv / std::min(nv, A.prec == F64 ? 2 : 4)));
+        const std::string tail = std::to_string(nv) + "," + op_name(op) + (wide ? ",16>" : ">");
         led.kernel = st7    ? std::string("k_spmv_st7<") + prec_name(A.prec) + "," + tail
                      : grid ? std::string("k_spmv_grid<") + prec_name(A.prec) + "," + tail
                             : std::string("k_spmv<") + prec_name(A.prec) + "," +

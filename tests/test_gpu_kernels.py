@@ -63,6 +63,74 @@ def test_spmv_irregular_rows_bit_exact(gpu, O, nv):
     assert np.array_equal(dy.to_host(), O.spmv(row, col, val, x, nv, ncols=m))
 
 
+@pytest.mark.parametrize("nv", [8, 32])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_spmv_wide_batches_bit_exact(gpu, O, nv, prec):
+    # few rows of many entries (the coarse levels of an aggressively coarsened hierarchy)
+    # take the 16-wide batches: the same sums in the same order as the 8-wide kernel and,
+    # in fp64, as the oracle; y = A x and y += A x, u16 and u32 columns, empty rows
+    X = gpu
+    rng = np.random.default_rng(nv * 7 + prec)
+    n = 2000
+    row, col, val = random_csr(rng, n, n, 0.04)
+    lens = np.diff(row).astype(np.int64)
+    lens[::11] = 0
+    keep = np.concatenate([np.arange(row[i], row[i] + lens[i]) for i in range(n)]).astype(np.int64)
+    col, val = col[keep], val[keep]
+    row = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    assert len(col) >= 32 * n
+    dt = np.float64 if prec == 0 else np.float32
+    x = rng.standard_normal(n * nv)
+    y0 = np.linspace(-1, 1, n * nv)
+    try:
+        for index_mode in (1, 2):
+            A = X.Matrix(row, col, val, prec=prec, index_mode=index_mode)
+            out = []
+            for wide in (0, 1):
+                X.set_option("spmv_wide_batches", wide)
+                dx = X.DeviceArray.from_host(x.astype(dt))
+                dy = X.DeviceArray(n * nv, dt)
+                A.spmv(nv, dx, dy)
+                dz = X.DeviceArray.from_host(y0.astype(dt))
+                A.spmv(nv, dx, dz, accumulate=True)
+                out.append((dy.to_host(), dz.to_host()))
+            assert np.array_equal(out[0][0], out[1][0]), index_mode
+            assert np.array_equal(out[0][1], out[1][1]), index_mode
+            if prec == 0:
+                assert np.array_equal(out[1][0], O.spmv(row, col, val, x, nv))
+    finally:
+        X.set_option("spmv_wide_batches", 1)
+
+
+def test_wide_batches_in_the_coarse_levels(gpu):
+    # the coarse levels of the aggressively coarsened 64^3 cube (levels 2 and 3: 2295 and
+    # 587 rows of 34 and 45 entries) take the 16-wide SpMVs; the preconditioner is
+    # bit-identical with them off
+    X = gpu
+    row, col, val = X.cube(64)
+    levels = X.Amg(row, col, val, agg_num_levels=2, num_paths=2, coarse_size=500).export()
+    n = len(row) - 1
+    prm = X.default_params(method="MergedPBiCGStab", smoother="Chebyshev", poly_order=4)
+    try:
+        for nv in (8, 32):
+            b = X.rhs_uniform(nv, n, nv)
+            out, names = [], []
+            for wide in (0, 1):
+                X.set_option("spmv_wide_batches", wide)
+                S = X.Solver(X.Hier(levels), prm, nv)
+                db, dx = X.DeviceArray.from_host(b), X.DeviceArray(n * nv)
+                S.precond_apply(db, dx)
+                out.append(dx.to_host())
+                dx.zero()
+                names.append({(k["kernel"], k["level"]) for k in S.profile(db, dx, 1)})
+            assert np.array_equal(out[0], out[1]), nv
+            assert not any(k.endswith(",16>") for k, _ in names[0])
+            assert any(k.endswith(",16>") and lv >= 2 for k, lv in names[1]), names[1]
+            assert not any(k.endswith(",16>") and lv == 0 for k, lv in names[1])
+    finally:
+        X.set_option("spmv_wide_batches", 1)
+
+
 def test_spmv_order_sensitive_row(gpu):
     # a row whose value depends on the summation order: (2^53 + 1) - 2^53 = 0 in the
     # reference's stored order
