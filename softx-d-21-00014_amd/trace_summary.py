@@ -31,7 +31,11 @@ def short(name):
     k, args = m.group(1), [a.strip() for a in m.group(2).split(",")]
     args = ["u16" if a == "unsigned short" else "u32" if a == "unsigned int" else a
             for a in args]
-    if k.startswith("k_spmv") and args[-1].isdigit():
+    if k == "k_spmv" and len(args) == 5 and args[3].isdigit():
+        # k_spmv<VT, IT, NV, OP, BX>: the x loads per batch shown only when not 8, as the
+        # solver's launch accounting names the kernel
+        args = args[:3] + [OPS["k_spmv"][int(args[3])]] + ([args[4]] if args[4] != "8" else [])
+    elif k.startswith("k_spmv") and args[-1].isdigit():
         args[-1] = OPS["k_spmv"][int(args[-1])]
     if k == "k_vec" and args[-1].isdigit():
         args[-1] = VEC[int(args[-1])]
