@@ -532,6 +532,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         const size_t oy = (OP == OP_SET || OP == OP_ADD) && rind != nullptr
                               ? (size_t)rind[i] * NV + k0
                               : o;
+        // the epilogue's operands, issued before the row's x loads so that both are in
+        // flight together (nv = 8: 146 -> 140 us per finest Chebyshev step), except in
+        // launches with dot partials, whose fewer workgroups of many tiles lost by it
+        // (OP_DOT1 425 -> 442 us)
+        EpiIn<VT, W> in;
+        if constexpr (ND == 0)
+            epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last,
+                                    x_zero);
         VT acc[W];
         if constexpr (OP == OP_ADD) {
             const P y0 = ld<VT, W>((const VT *)yv + oy);
@@ -555,8 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
         } else {
             row_acc<VT, IT, NV, W, false, false, BX>(jb, je, ch, col, val, base, x, k0, acc);
         }
-        EpiIn<VT, W> in;
-        epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last, x_zero);
+        if constexpr (ND > 0)
+            epi_load<VT, NV, W, OP>(i, o, oy, k0, in, yv, b, o1, isd, rr, out2, xupdv, last,
+                                    x_zero);
         row_epilogue<VT, NV, W, OP>(o, oy, acc, part, in, yv, out2, out3, xupdv, c, last,
                                     x_zero, out_f32);
         jb = nb;
